@@ -1,0 +1,388 @@
+"""Network descriptions for the lazy-update path: prototxt, stimulus and
+connection-map writers, and the benchmark workloads.
+
+The files follow the reference's formats:
+
+* the ``SolverParam`` text format of ``src/gsbn/proto/gsbn.proto:4-98``
+  (``gen_param``/``net_param``/``rec_param``/``proc_param``);
+* the ``StimRawData`` protobuf wire format (``gsbn.proto:153-160``):
+  ``data_rows, data_cols, packed uint32 data, mask_rows, mask_cols, packed
+  float mask``; ``0x7fffffff`` marks an unclamped hypercolumn, as written by
+  ``tools/stimgen/gen_rnd.py:61-78``;
+* the connection map read by ``ProcInitConn`` (``ProcInitConn.cpp:48-75``):
+  ``prj_id,dest_hcu,src_mcu,delay[,src_mcu,delay...]``, slots filled in file
+  order.
+
+Workloads (times in steps of dt = 1 ms):
+
+``recall_net(hcu, mcu, ...)`` builds one population with a dense recurrent
+projection without self-hypercolumn connections (slot k of destination
+hypercolumn h holds the k-th source unit outside h, ascending, delay 1),
+``n_pat`` random one-MCU-per-HCU patterns trained for ``train`` steps each
+followed by ``quiet`` unclamped steps, then ``n_pat`` cues with a fraction of
+the hypercolumns clamped, recalled for ``recall`` steps each with the
+recurrent support enabled (``wmask = 1``) on the unclamped hypercolumns and
+scored by ProcCheck.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import os
+import struct
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+UNCLAMPED = 0x7FFFFFFF
+
+
+@dataclasses.dataclass
+class PopSpec:
+    hcu: int
+    mcu: int
+    taum: float = 0.01
+    wtagain: float = 4.0
+    maxfq: float = 100.0
+    igain: float = 1.0
+    wgain: float = 1.0
+    lgbias: float = 0.0
+    snoise: float = 0.0
+
+
+@dataclasses.dataclass
+class ProjSpec:
+    src: int
+    dest: int
+    tauzi: float
+    tauzj: float
+    taue: float
+    taup: float
+    maxfq: float = 100.0
+    bgain: float = 1.0
+    wgain: float = 1.0
+    tauepsc: Optional[float] = None
+    slot_num: float = 0.0
+
+
+@dataclasses.dataclass
+class ModeSpec:
+    begin_time: float
+    end_time: float
+    time_step: int = 1
+    begin_lgidx_id: int = 0
+    begin_lgexp_id: int = 0
+    begin_wmask_id: int = 0
+    lgidx_step: int = 1
+    lgexp_step: int = 1
+    wmask_step: int = 1
+    prn: float = 1.0
+    plasticity: int = 0
+
+
+@dataclasses.dataclass
+class NetSpec:
+    dt: float
+    pops: List[PopSpec]
+    projs: List[ProjSpec]
+    modes: List[ModeSpec]
+    stim: np.ndarray  # uint32 [rows, total_hcu]
+    mask: np.ndarray  # float32 [rows, total_hcu]
+    conn: List[List[Tuple[int, List[Tuple[int, int]]]]]  # per proj: (dest_hcu, [(src, delay)])
+    eps: float = 0.001
+    seed: int = 0
+    wta_pop_max: int = 0
+    check_threshold: int = 0
+    ext_spikes: Optional[Dict[int, Dict[int, List[int]]]] = None  # pop -> step -> units
+    name: str = "net"
+
+    @property
+    def total_hcu(self) -> int:
+        return sum(p.hcu for p in self.pops)
+
+    @property
+    def total_units(self) -> int:
+        return sum(p.hcu * p.mcu for p in self.pops)
+
+
+def step_time(step: int, dt: float) -> float:
+    """A time in seconds that ProcExtGen maps to ``step`` (ProcExtGen.cpp:22-23
+    truncates ``float(time)/float(dt)``; half a step keeps the quotient away
+    from an integer)."""
+    return (step + 0.5) * dt
+
+
+def _varint(v: int) -> bytes:
+    out = bytearray()
+    while True:
+        b = v & 0x7F
+        v >>= 7
+        if v:
+            out.append(b | 0x80)
+        else:
+            out.append(b)
+            return bytes(out)
+
+
+def encode_stim(stim: np.ndarray, mask: np.ndarray) -> bytes:
+    """StimRawData wire encoding (gsbn.proto:153-160), fields in number order."""
+    stim = np.asarray(stim, dtype=np.uint32)
+    mask = np.asarray(mask, dtype=np.float32)
+    out = bytearray()
+    out += b"\x08" + _varint(stim.shape[0])
+    out += b"\x10" + _varint(stim.shape[1])
+    payload = b"".join(_varint(int(v)) for v in stim.reshape(-1))
+    out += b"\x1a" + _varint(len(payload)) + payload
+    out += b"\x20" + _varint(mask.shape[0])
+    out += b"\x28" + _varint(mask.shape[1])
+    fpay = mask.reshape(-1).astype("<f4").tobytes()
+    out += b"\x32" + _varint(len(fpay)) + fpay
+    return bytes(out)
+
+
+def decode_stim(buf: bytes):
+    """Inverse of encode_stim for the field subset StimRawData uses."""
+    pos = 0
+    fields: Dict[int, object] = {}
+    data: List[int] = []
+    mask: List[float] = []
+
+    def rd_varint(p):
+        shift = 0
+        val = 0
+        while True:
+            b = buf[p]
+            p += 1
+            val |= (b & 0x7F) << shift
+            shift += 7
+            if not b & 0x80:
+                return val, p
+
+    while pos < len(buf):
+        tag, pos = rd_varint(pos)
+        fno, wt = tag >> 3, tag & 7
+        if wt == 0:
+            v, pos = rd_varint(pos)
+            if fno == 3:
+                data.append(v)
+            else:
+                fields[fno] = v
+        elif wt == 2:
+            ln, pos = rd_varint(pos)
+            end = pos + ln
+            if fno == 3:
+                while pos < end:
+                    v, pos = rd_varint(pos)
+                    data.append(v)
+            elif fno == 6:
+                mask.extend(struct.unpack("<%df" % (ln // 4), buf[pos:end]))
+                pos = end
+            else:
+                pos = end
+        elif wt == 5:
+            if fno == 6:
+                mask.append(struct.unpack("<f", buf[pos:pos + 4])[0])
+            pos += 4
+        else:
+            raise ValueError("unsupported wire type %d" % wt)
+    drows, dcols = fields.get(1, 0), fields.get(2, 0)
+    mrows, mcols = fields.get(4, 0), fields.get(5, 0)
+    return (np.array(data, dtype=np.uint32).reshape(drows, dcols),
+            np.array(mask, dtype=np.float32).reshape(mrows, mcols))
+
+
+def _fmt(v: float) -> str:
+    return repr(float(v))
+
+
+def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
+             ext_spike_path: Optional[str] = None) -> str:
+    """SolverParam text (gsbn.proto:4-98) with the procedure list
+    ProcExtGen, ProcUpdLazy, ProcInitConn, ProcExchangeSpike, ProcCheck."""
+    L = []
+    L.append("gen_param {")
+    L.append('  stim_file: "%s"' % stim_path)
+    L.append("  dt: %s" % _fmt(spec.dt))
+    L.append("  eps: %s" % _fmt(spec.eps))
+    for m in spec.modes:
+        L.append("  mode_param {")
+        for f in dataclasses.fields(m):
+            v = getattr(m, f.name)
+            L.append("    %s: %s" % (f.name, _fmt(v) if isinstance(v, float) else v))
+        L.append("  }")
+    L.append("}")
+    L.append("net_param {")
+    for p in spec.pops:
+        L.append("  pop_param {")
+        L.append("    pop_num: 1")
+        L.append("    hcu_num: %d" % p.hcu)
+        L.append("    mcu_num: %d" % p.mcu)
+        L.append("    fanout_num: %d" % p.mcu)
+        for f in ("taum", "wtagain", "maxfq", "igain", "wgain", "lgbias", "snoise"):
+            L.append("    %s: %s" % (f, _fmt(getattr(p, f))))
+        L.append("    rank: 0")
+        L.append("  }")
+    for q in spec.projs:
+        L.append("  proj_param {")
+        L.append("    src_pop: %d" % q.src)
+        L.append("    dest_pop: %d" % q.dest)
+        for f in ("tauzi", "tauzj", "taue", "taup", "maxfq", "bgain", "wgain"):
+            L.append("    %s: %s" % (f, _fmt(getattr(q, f))))
+        if q.tauepsc is not None:
+            L.append("    tauepsc: %s" % _fmt(q.tauepsc))
+        L.append("    slot_num: %s" % _fmt(q.slot_num))
+        L.append("  }")
+    L.append("}")
+    L.append('rec_param { directory: "%s" }' % rec_dir)
+    L.append('proc_param { name: "ProcExtGen" }')
+    L.append("proc_param {")
+    L.append('  name: "ProcUpdLazy"')
+    L.append('  argi { key: "seed-lo" val: %d }' % (spec.seed & 0x7FFFFFFF))
+    L.append('  argi { key: "seed-hi" val: %d }' % ((spec.seed >> 31) & 0x7FFFFFFF))
+    L.append('  argi { key: "wta-pop-max" val: %d }' % spec.wta_pop_max)
+    if ext_spike_path:
+        L.append('  args { key: "external spike" val: "%s" }' % ext_spike_path)
+    L.append("}")
+    L.append("proc_param {")
+    L.append('  name: "ProcInitConn"')
+    L.append('  args { key: "conn-map-file" val: "%s" }' % conn_path)
+    L.append("}")
+    L.append('proc_param { name: "ProcExchangeSpike" }')
+    L.append("proc_param {")
+    L.append('  name: "ProcCheck"')
+    L.append('  argi { key: "threashold" val: %d }' % spec.check_threshold)
+    L.append("}")
+    return "\n".join(L) + "\n"
+
+
+def conn_map_text(spec: NetSpec) -> str:
+    lines = []
+    for pid, rows in enumerate(spec.conn):
+        for dest_hcu, pairs in rows:
+            if not pairs:
+                continue
+            parts = [str(pid), str(dest_hcu)]
+            for s, d in pairs:
+                parts.append(str(s))
+                parts.append(str(d))
+            lines.append(",".join(parts))
+    return "\n".join(lines) + "\n"
+
+
+def write(spec: NetSpec, directory: str) -> str:
+    """Write prototxt + stimulus + connection map (+ external spikes) into
+    ``directory``; return the prototxt path."""
+    os.makedirs(directory, exist_ok=True)
+    stim_path = os.path.join(directory, spec.name + "_stim.bin")
+    conn_path = os.path.join(directory, spec.name + "_conn.map")
+    rec_dir = os.path.join(directory, spec.name + "_rec")
+    with open(stim_path, "wb") as f:
+        f.write(encode_stim(spec.stim, spec.mask))
+    with open(conn_path, "w") as f:
+        f.write(conn_map_text(spec))
+    ext_path = None
+    if spec.ext_spikes:
+        ext_path = os.path.join(directory, spec.name + "_ext.csv")
+        with open(ext_path, "w") as f:
+            for pop, steps in spec.ext_spikes.items():
+                for step in sorted(steps):
+                    units = steps[step]
+                    f.write(",".join([str(step), str(pop)] + [str(u) for u in units]) + "\n")
+    path = os.path.join(directory, spec.name + ".prototxt")
+    with open(path, "w") as f:
+        f.write(prototxt(spec, stim_path, conn_path, rec_dir, ext_path))
+    return path
+
+
+def dense_conn(hcu: int, mcu: int, delay: int = 1, self_hcu: bool = False):
+    """Dense recurrent map: destination HCU h receives every source unit outside
+    h (or every unit if self_hcu), ascending, all with the same delay."""
+    rows = []
+    n = hcu * mcu
+    for h in range(hcu):
+        pairs = [(s, delay) for s in range(n) if self_hcu or s // mcu != h]
+        rows.append((h, pairs))
+    return rows
+
+
+def random_patterns(rng: np.random.Generator, n_pat: int, hcu: int, mcu: int) -> np.ndarray:
+    """Distinct one-MCU-per-HCU patterns (tools/stimgen/gen_rnd.py:38-58)."""
+    pats: List[List[int]] = []
+    seen = set()
+    while len(pats) < n_pat:
+        p = [int(v) for v in rng.integers(0, mcu, size=hcu)]
+        t = tuple(p)
+        if t in seen:
+            continue
+        seen.add(t)
+        pats.append(p)
+    return np.array(pats, dtype=np.uint32)
+
+
+def recall_net(hcu: int, mcu: int, n_pat: int, train: int, quiet: int, recall: int,
+               clamp_frac: float, *, seed: int = 0, stim_seed: int = 0, snoise: float = 0.1,
+               wtagain: float = 4.0, pop_wgain: float = 8.0, tauzi: float = 0.01,
+               tauzj: float = 0.01,
+               taue: float = 0.02, taup: float = 1.0, tauepsc: Optional[float] = None,
+               delay: int = 1, dt: float = 0.001, reset: int = 0, reset_prn: float = 1e-30,
+               name: str = "net") -> NetSpec:
+    rng = np.random.Generator(np.random.Philox(stim_seed))
+    pats = random_patterns(rng, n_pat, hcu, mcu)
+    quiet_row = np.full((1, hcu), UNCLAMPED, dtype=np.uint32)
+    cues = pats.copy()
+    masks = np.zeros((n_pat, hcu), dtype=np.float32)
+    n_clamp = int(round(clamp_frac * hcu))
+    for i in range(n_pat):
+        clamped = set(int(v) for v in rng.permutation(hcu)[:n_clamp])
+        for h in range(hcu):
+            if h not in clamped:
+                cues[i, h] = UNCLAMPED
+                masks[i, h] = 1.0
+    stim = np.concatenate([pats, quiet_row, cues], axis=0)
+    mask = np.concatenate([np.zeros((1, hcu), dtype=np.float32), masks], axis=0)
+    modes: List[ModeSpec] = []
+    t = 0
+    for i in range(n_pat):
+        modes.append(ModeSpec(step_time(t, dt), step_time(t + train, dt), time_step=train,
+                              begin_lgidx_id=i, begin_wmask_id=0, prn=1.0))
+        t += train
+        if quiet > 0:
+            modes.append(ModeSpec(step_time(t, dt), step_time(t + quiet, dt), time_step=quiet,
+                                  begin_lgidx_id=n_pat, begin_wmask_id=0, prn=1.0))
+            t += quiet
+    if reset <= 0:
+        modes.append(ModeSpec(step_time(t, dt), step_time(t + n_pat * recall, dt),
+                              time_step=recall, begin_lgidx_id=n_pat + 1, lgidx_step=1,
+                              begin_wmask_id=1, wmask_step=1, begin_lgexp_id=0, lgexp_step=1,
+                              prn=0.0))
+    else:
+        # each cue is preceded by `reset` unclamped steps without recurrent
+        # support and with a negligible, non-zero learning rate, so that the
+        # previous attractor decays and ProcCheck (which scores prn == 0
+        # modes only) does not score the reset
+        for i in range(n_pat):
+            modes.append(ModeSpec(step_time(t, dt), step_time(t + reset, dt), time_step=reset,
+                                  begin_lgidx_id=n_pat, begin_wmask_id=0, prn=reset_prn))
+            t += reset
+            modes.append(ModeSpec(step_time(t, dt), step_time(t + recall, dt), time_step=recall,
+                                  begin_lgidx_id=n_pat + 1 + i, begin_wmask_id=1 + i,
+                                  begin_lgexp_id=i, prn=0.0))
+            t += recall
+    pop = PopSpec(hcu=hcu, mcu=mcu, wtagain=wtagain, wgain=pop_wgain, snoise=snoise)
+    proj = ProjSpec(src=0, dest=0, tauzi=tauzi, tauzj=tauzj, taue=taue, taup=taup,
+                    tauepsc=tauepsc, slot_num=float((hcu - 1) * mcu))
+    return NetSpec(dt=dt, pops=[pop], projs=[proj], modes=modes, stim=stim, mask=mask,
+                   conn=[dense_conn(hcu, mcu, delay)], seed=seed, name=name)
+
+
+def config0(seed: int = 0) -> NetSpec:
+    """10 HCU x 10 MCU, 10 patterns x 100 steps + 10 quiet steps, 10 cues with
+    50% of the HCUs clamped, 100 recall steps each after a 50-step reset."""
+    return recall_net(10, 10, 10, 100, 10, 100, 0.5, seed=seed, reset=50, name="cfg0")
+
+
+def config1(seed: int = 0) -> NetSpec:
+    """100 HCU x 100 MCU (9.9e7 synapses), 100 patterns x 50 steps + 10 quiet
+    steps, 100 cues with 30% of the HCUs clamped, 50 recall steps each after a
+    50-step reset."""
+    return recall_net(100, 100, 100, 50, 10, 50, 0.3, seed=seed, reset=50, name="cfg1")

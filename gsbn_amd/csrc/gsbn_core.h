@@ -1,0 +1,100 @@
+/*
+ * gsbn_core.h -- state layout and the per-element update rules of the lazy
+ * BCPNN procedure, shared by the HIP kernels and the host code of this library.
+ *
+ * Per-synapse state.  The reference keeps six row-major arrays
+ * pij/eij/zi2/zj2/wij/tij indexed (h*C + k)*M + j (Proj.cpp:99-104).  Row
+ * updates walk j for fixed (h,k) and column updates walk k for fixed (h,j), so
+ * in separate arrays one of the two is always a stride-M walk that touches five
+ * different cache lines per synapse.  Here the six values of one synapse form
+ * one 32-byte record, records are stored in the same (h*C + k)*M + j order, and
+ * a column update reads and writes one aligned 32-byte sector per synapse
+ * while a row update streams M consecutive records.
+ *
+ * Per-row state (pi, ei, zi, ti; Proj.cpp:91-94) is one 16-byte record.
+ *
+ * Closed-form catch-up.  Over an elapsed gap of D whole steps the traces obey
+ * the linear cascade dz/dt = -kz z, de/dt = ke (z - e), dp/dt = kp (e - p)
+ * (Proj.cpp:241-247 for the row traces, Proj.cpp:269-277 for the synapses, with
+ * kz = kzi for rows and kz = kzi + kzj for the product zi2*zj2).  Its solution
+ * is linear in the state:
+ *     z' = Ez z
+ *     e' = Ee e + Cez z
+ *     p' = Ep p + Cpe e + Cpz z
+ * with Ee = exp(-ke D), Ep = exp(-kp D), Cez = ke phi(kz,ke), Cpe = kp phi(ke,kp),
+ * Cpz = kp ke psi(kz,ke,kp), phi and psi being the first and second divided
+ * differences of x -> exp(-x D) (sign chosen so that both are positive).  The
+ * coefficients are tabulated per D in double precision on the host (coeff.cpp)
+ * and rounded to float; each update is then a fixed sequence of float
+ * multiplies and adds evaluated in the order written below.
+ */
+#ifndef GSBN_CORE_H
+#define GSBN_CORE_H
+
+#include <stdint.h>
+#include "detmath.h"
+
+typedef struct {
+  float pij, eij, zi2, zj2, wij;
+  int32_t tij;
+  int32_t pad0, pad1;
+} gsbn_syn_t;
+
+typedef struct {
+  float pi, ei, zi;
+  int32_t ti;
+} gsbn_rowst_t;
+
+/* coefficient table: GSBN_COEF_STRIDE floats per elapsed gap D */
+#define GSBN_COEF_STRIDE 12
+enum {
+  GSBN_C_EZI = 0,   /* exp(-kzi D) */
+  GSBN_C_EZJ = 1,   /* exp(-kzj D) */
+  GSBN_C_EE = 2,    /* exp(-ke D) */
+  GSBN_C_EP = 3,    /* exp(-kp D) */
+  GSBN_C_EZ_IJ = 4, /* ke phi(kzi+kzj, ke) */
+  GSBN_C_PE = 5,    /* kp phi(ke, kp) */
+  GSBN_C_PZ_IJ = 6, /* kp ke psi(kzi+kzj, ke, kp) */
+  GSBN_C_EZ_I = 7,  /* ke phi(kzi, ke) */
+  GSBN_C_PZ_I = 8   /* kp ke psi(kzi, ke, kp) */
+};
+
+/* value written into the support-contribution buffer for a synapse whose row
+ * update took the D<=0 branch (Proj.cpp:428-431 adds nothing to epsc) */
+#define GSBN_NO_CONTRIB_BITS 0x7fbadbadu
+
+/* Synapse catch-up over a gap with coefficients c (pointer to one table
+ * entry).  Row form adds kfti to zi2, column form adds kftj to zj2. */
+GSBN_HD GSBN_INLINE void gsbn_syn_catchup(const float* c, float* pij, float* eij,
+                                          float* zi2, float* zj2) {
+  float z0 = (*zi2) * (*zj2);
+  float p = (c[GSBN_C_EP] * (*pij) + c[GSBN_C_PE] * (*eij)) + c[GSBN_C_PZ_IJ] * z0;
+  float e = c[GSBN_C_EE] * (*eij) + c[GSBN_C_EZ_IJ] * z0;
+  *pij = p;
+  *eij = e;
+  *zi2 = c[GSBN_C_EZI] * (*zi2);
+  *zj2 = c[GSBN_C_EZJ] * (*zj2);
+}
+
+GSBN_HD GSBN_INLINE void gsbn_row_catchup(const float* c, float* pi, float* ei, float* zi) {
+  float p = (c[GSBN_C_EP] * (*pi) + c[GSBN_C_PE] * (*ei)) + c[GSBN_C_PZ_I] * (*zi);
+  float e = c[GSBN_C_EE] * (*ei) + c[GSBN_C_EZ_I] * (*zi);
+  *pi = p;
+  *ei = e;
+  *zi = c[GSBN_C_EZI] * (*zi);
+}
+
+/* Wij = wgain * log((pij + eps^2) / ((pi + eps) (pj + eps)))  (Proj.cpp:461) */
+GSBN_HD GSBN_INLINE float gsbn_wij(float pij, float pi, float pj, float wgain, float eps,
+                                   float eps2) {
+  return wgain * gsbn_logf((pij + eps2) / ((pi + eps) * (pj + eps)));
+}
+
+/* table lookup: gaps beyond the table use the last entry, which the builder
+ * only allows when that entry is the exact saturated value */
+GSBN_HD GSBN_INLINE const float* gsbn_coef(const float* table, int len, int gap) {
+  int d = gap < len ? gap : len - 1;
+  return table + (int64_t)d * GSBN_COEF_STRIDE;
+}
+
+#endif /* GSBN_CORE_H */

@@ -1,0 +1,402 @@
+"""Python driver of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this module.  It wraps liboracle.so (the C restatement of
+ProcUpdLazy, see gsbn_oracle.h) and restates in Python the two small host
+procedures around it:
+
+* ``ExtGen``: ProcExtGen's mode schedule (ProcExtGen.cpp:8-107 init,
+  :113-153 update): chunking of each mode_param into ``time_step`` pieces,
+  ``simstep`` increment, ``prn``/``old-prn``/``wmask-idx``/``cycle-flag``
+  updates and the ``.lginp`` rebuild (``log(1+eps)`` for the clamped MCU,
+  ``log(eps)`` otherwise).
+* ``Check``: ProcCheck (ProcCheck.cpp:8-143 init, :149-237 update;
+  ProcCheck/Pop.cpp:35-60): per-window spike counts for prn==0 modes, first
+  strict argmax per HCU, the correctness criterion of ProcCheck.cpp:196.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Dict, List, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ARITH_TABLE, ARITH_REF32, ARITH_F64 = 0, 1, 2
+_DT = {0: np.int8, 2: np.int32, 5: np.float32, 6: np.float64}
+
+
+class PopCfg(ctypes.Structure):
+    _fields_ = [("hcu", ctypes.c_int), ("mcu", ctypes.c_int), ("taum", ctypes.c_float),
+                ("wtagain", ctypes.c_float), ("maxfq", ctypes.c_float),
+                ("igain", ctypes.c_float), ("wgain", ctypes.c_float),
+                ("lgbias", ctypes.c_float), ("snoise", ctypes.c_float)]
+
+
+class ProjCfg(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_int), ("dest", ctypes.c_int), ("tauzi", ctypes.c_float),
+                ("tauzj", ctypes.c_float), ("taue", ctypes.c_float), ("taup", ctypes.c_float),
+                ("maxfq", ctypes.c_float), ("bgain", ctypes.c_float),
+                ("wgain", ctypes.c_float), ("tauepsc", ctypes.c_float),
+                ("has_tauepsc", ctypes.c_int), ("slot_num", ctypes.c_float)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, f32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int64
+        L.gor_net_new.restype = vp
+        L.gor_net_new.argtypes = [f32, i32, ctypes.POINTER(PopCfg), i32, ctypes.POINTER(ProjCfg),
+                                  ctypes.c_uint64, i32, i32]
+        L.gor_net_free.argtypes = [vp]
+        L.gor_set_globals.argtypes = [vp, i32, f32, f32, i32, i32, i32]
+        L.gor_set_inputs.argtypes = [vp, vp, i32, vp, i32]
+        L.gor_set_ext_spikes.argtypes = [vp, i32, i32, vp, vp]
+        L.gor_updlazy_step.argtypes = [vp]
+        L.gor_exchange_spikes.argtypes = [vp]
+        L.gor_proj_step_replay.argtypes = [vp, i32, vp]
+        L.gor_vector.restype = i64
+        L.gor_vector.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), vp, i64]
+        L.gor_vector_write.restype = i32
+        L.gor_vector_write.argtypes = [vp, ctypes.c_char_p, vp, i64]
+        L.gor_coeff_entry.argtypes = [ctypes.c_double] * 4 + [i64, vp]
+        L.gor_closed_form_f64.argtypes = [ctypes.c_double] * 3 + [i64, vp, vp]
+        L.gor_ref32_row.argtypes = [f32, f32, f32, i32, vp, vp, vp]
+        L.gor_ref32_syn.argtypes = [f32, f32, f32, f32, i32, vp, vp, vp, vp]
+        L.gor_philox4x32_10.argtypes = [vp, vp, vp]
+        L.gor_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.c_uint32, vp, vp]
+        L.gor_work_counters.argtypes = [vp, vp]
+        L.gor_det_logf.restype = f32
+        L.gor_det_logf.argtypes = [f32]
+        L.gor_det_expf.restype = f32
+        L.gor_det_expf.argtypes = [f32]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().gor_philox4x32_10(_p(c), _p(k), _p(o))
+    return o
+
+
+def draw(seed: int, pop: int, step: int, unit: int):
+    u = np.zeros(1, np.float32)
+    z = np.zeros(1, np.float32)
+    lib().gor_draw(seed, pop, step, unit, _p(u), _p(z))
+    return float(u[0]), float(z[0])
+
+
+def det_logf(x) -> np.float32:
+    return np.float32(lib().gor_det_logf(np.float32(x)))
+
+
+def det_expf(x) -> np.float32:
+    return np.float32(lib().gor_det_expf(np.float32(x)))
+
+
+def coeff_entry(kzi, kzj, ke, kp, gap) -> np.ndarray:
+    o = np.zeros(12, np.float32)
+    lib().gor_coeff_entry(float(kzi), float(kzj), float(ke), float(kp), int(gap), _p(o))
+    return o
+
+
+def closed_form_f64(kz, ke, kp, gap, state) -> np.ndarray:
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    o = np.zeros(3, np.float64)
+    lib().gor_closed_form_f64(float(kz), float(ke), float(kp), int(gap), _p(s), _p(o))
+    return o
+
+
+def ref32_row(kp, ke, kzi, pdt, pi, ei, zi):
+    a = np.array([pi, ei, zi], dtype=np.float32)
+    lib().gor_ref32_row(kp, ke, kzi, int(pdt), _p(a[0:]), _p(a[1:]), _p(a[2:]))
+    return a
+
+
+def ref32_syn(kp, ke, kzi, kzj, pdt, pij, eij, zi2, zj2):
+    a = np.array([pij, eij, zi2, zj2], dtype=np.float32)
+    lib().gor_ref32_syn(kp, ke, kzi, kzj, int(pdt), _p(a[0:]), _p(a[1:]), _p(a[2:]), _p(a[3:]))
+    return a
+
+
+class ExtGen:
+    """ProcExtGen restated (ProcExtGen.cpp:8-153)."""
+
+    def __init__(self, spec):
+        dt = np.float32(spec.dt)
+        self.eps = np.float32(spec.eps)
+        self.modes: List[dict] = []
+        for m in spec.modes:
+            begin = int(np.float32(m.begin_time) / dt)
+            end = int(np.float32(m.end_time) / dt)
+            lg, lx, wm = m.begin_lgidx_id, m.begin_lgexp_id, m.begin_wmask_id
+            while begin < end:
+                e = begin + m.time_step
+                if e >= end:
+                    e = end
+                self.modes.append(dict(begin=begin, end=e, lgidx=lg, lgexp=lx, wmask=wm,
+                                       prn=float(np.float32(m.prn))))
+                lg += m.lgidx_step
+                lx += m.lgexp_step
+                wm += m.wmask_step
+                begin = e
+        self.stim = np.asarray(spec.stim, dtype=np.uint32).astype(np.int64)
+        self.mask = np.ascontiguousarray(spec.mask, dtype=np.float32)
+        self.mcu_in_hcu = [p.mcu for p in spec.pops for _ in range(p.hcu)]
+        self.lginp = np.zeros(sum(self.mcu_in_hcu), dtype=np.float32)
+        self.cursor = 0
+        self.old_lgidx = -1
+        self.simstep = 0
+        self.prn = 0.0
+        self.old_prn = 0.0
+        self.cycle_flag = 0
+        self.wmask_idx = 0
+        self.lginp_idx = 0
+        self.l_on = det_logf(np.float32(1) + self.eps)
+        self.l_off = det_logf(np.float32(0) + self.eps)
+
+    def update(self):
+        self.simstep += 1
+        step = self.simstep
+        while self.cursor < len(self.modes):
+            m = self.modes[self.cursor]
+            if m["begin"] < step <= m["end"]:
+                self.old_prn = self.prn
+                self.prn = m["prn"]
+                self.wmask_idx = m["wmask"]
+                self.lginp_idx = 0
+                self.cycle_flag = 1
+                if self.old_lgidx != m["lgidx"]:
+                    self.old_lgidx = m["lgidx"]
+                    row = self.stim[m["lgidx"]]
+                    k = 0
+                    for i, nm in enumerate(self.mcu_in_hcu):
+                        for j in range(nm):
+                            self.lginp[k] = self.l_on if row[i] == j else self.l_off
+                            k += 1
+                return
+            elif step <= m["begin"]:
+                self.cycle_flag = 0
+                return
+            self.cursor += 1
+        self.cycle_flag = -1
+
+
+class Check:
+    """ProcCheck restated (ProcCheck.cpp:8-237, ProcCheck/Pop.cpp:35-60)."""
+
+    def __init__(self, spec, threshold: int = 0):
+        dt = np.float32(spec.dt)
+        self.modes = []
+        for m in spec.modes:
+            if np.float32(m.prn) != 0.0:
+                continue
+            begin = int(np.float32(m.begin_time) / dt)
+            end = int(np.float32(m.end_time) / dt)
+            lx = m.begin_lgexp_id
+            while begin < end:
+                self.modes.append(dict(begin=begin, end=begin + m.time_step, lgexp=lx))
+                lx += m.lgexp_step
+                begin += m.time_step
+        self.stim = np.asarray(spec.stim, dtype=np.uint32).astype(np.int64)
+        self.hcu = [(p.hcu, p.mcu) for p in spec.pops]
+        self.counters = [np.zeros((h, m), dtype=np.int64) for h, m in self.hcu]
+        self.threshold = threshold
+        self.cursor = 0
+        self.updated = False
+        self.results: List[dict] = []
+        self.correct = 0
+        self.total = 0
+
+    def update(self, cycle_flag: int, simstep: int, spikes: List[np.ndarray]):
+        if cycle_flag == 0 or cycle_flag < 0:
+            return
+        if self.cursor >= len(self.modes):
+            return
+        m = self.modes[self.cursor]
+        if simstep <= m["begin"]:
+            return
+        if simstep < m["end"]:
+            for c, s in zip(self.counters, spikes):
+                c += (s.reshape(c.shape) > 0)
+            self.updated = True
+            return
+        if not self.updated:
+            while self.cursor < len(self.modes) and simstep >= self.modes[self.cursor]["end"]:
+                self.cursor += 1
+            return
+        idx_all, cnt_all = [], []
+        for c in self.counters:
+            for row in c:
+                mx_i, mx_c = -1, 0
+                for j, v in enumerate(row):
+                    if v > mx_c:
+                        mx_i, mx_c = j, int(v)
+                idx_all.append(mx_i)
+                cnt_all.append(mx_c)
+            c[:] = 0
+        exp = self.stim[m["lgexp"]]
+        exp = np.where(exp > 0x7FFFFFFF, exp - (1 << 32), exp)
+        ok = True
+        for i in range(len(idx_all)):
+            e = int(exp[i])
+            if ((cnt_all[i] < self.threshold or idx_all[i] != e) and e >= 0) or \
+                    (cnt_all[i] > self.threshold and e < 0):
+                ok = False
+                break
+        self.results.append(dict(ok=ok, expected=[int(v) for v in exp], idx=idx_all,
+                                 cnt=cnt_all))
+        self.total += 1
+        self.correct += int(ok)
+        self.cursor += 1
+
+
+class OracleNet:
+    """A network in the oracle, built from a gsbn_amd.configs.NetSpec."""
+
+    def __init__(self, spec, arith: int = ARITH_TABLE):
+        L = lib()
+        self.spec = spec
+        pops = (PopCfg * len(spec.pops))()
+        for i, p in enumerate(spec.pops):
+            pops[i] = PopCfg(p.hcu, p.mcu, p.taum, p.wtagain, p.maxfq, p.igain, p.wgain,
+                             p.lgbias, p.snoise)
+        projs = (ProjCfg * max(1, len(spec.projs)))()
+        for i, q in enumerate(spec.projs):
+            projs[i] = ProjCfg(q.src, q.dest, q.tauzi, q.tauzj, q.taue, q.taup, q.maxfq,
+                               q.bgain, q.wgain, q.tauepsc if q.tauepsc is not None else 0.0,
+                               1 if q.tauepsc is not None else 0, q.slot_num)
+        self.h = L.gor_net_new(spec.dt, len(spec.pops), pops, len(spec.projs), projs,
+                               spec.seed, spec.wta_pop_max, arith)
+        self.arith = arith
+        self.extgen = ExtGen(spec)
+        self.check = Check(spec, spec.check_threshold)
+        self._mask = np.ascontiguousarray(spec.mask, dtype=np.float32)
+        self._keep = []
+        self._init_conn()
+        if spec.ext_spikes:
+            for pop, steps in spec.ext_spikes.items():
+                nsteps = max(steps) + 1 if steps else 0
+                off = np.zeros(nsteps + 1, np.int32)
+                units: List[int] = []
+                for s in range(nsteps):
+                    units.extend(steps.get(s, []))
+                    off[s + 1] = len(units)
+                u = np.array(units, dtype=np.int32)
+                L.gor_set_ext_spikes(self.h, pop, nsteps, _p(off), _p(u))
+
+    def _init_conn(self):
+        """ProcInitConn::add_row (ProcInitConn.cpp:92-103): fill the first free
+        slot of the destination HCU, in file order."""
+        for pid, rows in enumerate(self.spec.conn):
+            ii = self.vector("ii_%d" % pid)
+            di = self.vector("di_%d" % pid)
+            q = self.spec.projs[pid]
+            dp = self.spec.pops[q.dest]
+            sp = self.spec.pops[q.src]
+            C = sp.hcu * sp.mcu
+            if C > q.slot_num:  # Proj.cpp:22-26
+                C = int(q.slot_num)
+            for dest_hcu, pairs in rows:
+                base = dest_hcu * C
+                for s, d in pairs:
+                    for k in range(C):
+                        if ii[base + k] < 0:
+                            ii[base + k] = s
+                            di[base + k] = d
+                            break
+            self.write("ii_%d" % pid, ii)
+            self.write("di_%d" % pid, di)
+            del dp
+
+    def vector(self, name: str) -> np.ndarray:
+        L = lib()
+        t = ctypes.c_int(0)
+        n = L.gor_vector(self.h, name.encode(), ctypes.byref(t), None, 0)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n, dtype=_DT[t.value])
+        if n:
+            L.gor_vector(self.h, name.encode(), ctypes.byref(t), _p(out), n)
+        return out
+
+    def write(self, name: str, arr: np.ndarray):
+        L = lib()
+        t = ctypes.c_int(0)
+        n = L.gor_vector(self.h, name.encode(), ctypes.byref(t), None, 0)
+        a = np.ascontiguousarray(arr, dtype=_DT[t.value])
+        rc = L.gor_vector_write(self.h, name.encode(), _p(a), a.size)
+        if rc != 0:
+            raise ValueError("write %s failed (%d, expected %d elements)" % (name, rc, n))
+
+    def step(self) -> int:
+        """One Upd::update: ExtGen, ProcUpdLazy, ProcExchangeSpike, ProcCheck.
+        Returns cycle-flag."""
+        L = lib()
+        eg = self.extgen
+        eg.update()
+        L.gor_set_globals(self.h, eg.simstep, eg.prn, eg.old_prn, eg.cycle_flag, eg.lginp_idx,
+                          eg.wmask_idx)
+        L.gor_set_inputs(self.h, _p(eg.lginp), 1, _p(self._mask), self._mask.shape[0])
+        if eg.cycle_flag == 1:
+            L.gor_updlazy_step(self.h)
+            L.gor_exchange_spikes(self.h)
+            spikes = [self.vector("spike_%d" % i) for i in range(len(self.spec.pops))]
+        else:
+            spikes = []
+        self.check.update(eg.cycle_flag, eg.simstep, spikes)
+        return eg.cycle_flag
+
+    def run(self, max_steps: Optional[int] = None, record: bool = False):
+        raster: Dict[int, np.ndarray] = {}
+        n = 0
+        while True:
+            f = self.step()
+            if f < 0:
+                break
+            if record and f == 1:
+                raster[self.extgen.simstep] = np.nonzero(self.vector("spike_0"))[0]
+            n += 1
+            if max_steps is not None and n >= max_steps:
+                break
+        return raster
+
+    def work(self) -> np.ndarray:
+        o = np.zeros(4, np.int64)
+        lib().gor_work_counters(self.h, _p(o))
+        return o
+
+    def close(self):
+        if self.h:
+            lib().gor_net_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
