@@ -1,0 +1,178 @@
+#!/usr/bin/env python
+"""Benchmark of the lazy-update hot path on one MI355X.
+
+A step is one simulated timestep (1 ms of model time) of the 100 HCU x 100
+MCU network with its dense recurrent projection (9.9e7 synapses, FP32 state):
+ProcExtGen, ProcUpdLazy (population update, row and column lazy updates, delay
+queue), ProcExchangeSpike and ProcCheck, exactly as the solver runs them.
+Synthetic input: seeded one-MCU-per-HCU patterns.  The timed steps are taken
+from the training phase after the warm-up steps (the first step carries the
+full catch-up sweep of the learning-rate switch and is part of the warm-up).
+
+Printed value: synapse-trace updates per second, counted as the reference's
+loops do (each active row updates M synapses, each active column C synapses,
+plus any full-sweep synapses), with simulated steps per second beside it.
+
+roofline: the row/column kernel, timed with HIP events on the solver stream
+around every launch in the timed region; algorithmic bytes are 44 per
+synapse update (read pij, eij, zi2, zj2, tij; write them and wij).
+
+cpu_baseline: this project's CPU restatement of the same procedure (the
+oracle, plain C, -O2, one thread pinned to one core) on the same workload:
+the same warm-up, then as many steps as fit in about 20 s.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("synapse-trace updates/sec + sim steps/sec at 100×100 HCU×MCU, plain vs "
+          "BF16/batched")
+BYTES_PER_SYN_UPDATE = 44
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(name: str):
+    from gsbn_amd import configs
+    if name == "100x100":
+        return configs.config1(), ("lazy BCPNN update, 100 HCU x 100 MCU, dense recurrent "
+                                   "projection (9.9e7 synapses), plain FP32 state")
+    if name == "10x10":
+        return configs.config0(), ("lazy BCPNN update, 10 HCU x 10 MCU, dense recurrent "
+                                   "projection (9e3 synapses), plain FP32 state")
+    raise SystemExit("unknown workload " + name)
+
+
+def cpu_baseline(args) -> dict:
+    """Run the oracle in a child process pinned to one core."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-only", "--workload", args.workload,
+           "--warmup", str(args.warmup), "--cpu-seconds", str(args.cpu_seconds)]
+    out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                         timeout=900)
+    if out.returncode != 0:
+        return {"value": None, "unit": "synapse-trace updates/s", "cores": 1, "kind": "port",
+                "sample": "failed: " + out.stderr.strip().splitlines()[-1][:200]}
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_only(args):
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    spec, _ = workload(args.workload)
+    net = O.OracleNet(spec)
+    for _ in range(args.warmup):
+        net.step()
+    w0 = net.work()
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        if net.step() < 0:
+            break
+        n += 1
+    dt = time.perf_counter() - t0
+    w1 = net.work()
+    upd = int((w1[0] - w0[0]) + (w1[1] - w0[1]) + (w1[2] - w0[2]))
+    print(json.dumps({"value": upd / dt, "unit": "synapse-trace updates/s", "cores": 1,
+                      "kind": "port", "steps_per_sec": n / dt,
+                      "sample": "%d steps after %d warm-up steps of the same workload, %.1f s, "
+                                "1 thread pinned to one core" % (n, args.warmup, dt)}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="100x100")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true")
+    args = ap.parse_args()
+    if args.cpu_only:
+        return cpu_only(args)
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU form is a design only "
+              "(see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from gsbn_amd import Solver, configs
+
+    spec, desc = workload(args.workload)
+    with tempfile.TemporaryDirectory() as d:
+        path = configs.write(spec, d)
+        t_init = time.perf_counter()
+        s = Solver(path)
+        s.sync()
+        t_init = time.perf_counter() - t_init
+        s.step(args.warmup)
+        s.sync()
+        s.timing(-1, collect=False)
+        w0 = s.work()
+        s.timing(1 << 1, collect=False)  # HIP events around every row/column launch
+        t0 = time.perf_counter()
+        done = s.step(args.steps)
+        s.sync()
+        t1 = time.perf_counter()
+        launches, ms = s.timing(0)
+        w1 = s.work()
+        s.close()
+    el = t1 - t0
+    rows = int(w1[0] - w0[0])
+    cols = int(w1[1] - w0[1])
+    sweep = int(w1[2] - w0[2])
+    updates = rows + cols + sweep
+    value = updates / el
+    rc_launch, rc_ms = int(launches[1]), float(ms[1])
+    avg_ms = rc_ms / max(rc_launch, 1)
+    bytes_per_launch = BYTES_PER_SYN_UPDATE * (rows + cols) / max(rc_launch, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "rowcol_100x100_traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "synapse-trace updates/s",
+        "n_gpus": 1,
+        "steps": int(done),
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / max(done, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "hcu": spec.pops[0].hcu, "mcu": spec.pops[0].mcu,
+                   "synapses": spec.pops[0].hcu * int(spec.projs[0].slot_num) * spec.pops[0].mcu,
+                   "phase": "training steps %d..%d" % (args.warmup + 1, args.warmup + done)},
+        "steps_per_sec": done / el,
+        "work": {"row_synapse_updates": rows, "col_synapse_updates": cols,
+                 "sweep_synapse_updates": sweep},
+        "init_s": t_init,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "rowcol_kernel", "avg_launch_ms": avg_ms,
+                     "launches": rc_launch, "bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

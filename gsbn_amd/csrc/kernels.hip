@@ -1,0 +1,775 @@
+/*
+ * kernels.hip -- CDNA4 (gfx950) kernels of the lazy-update BCPNN step and the
+ * kernel-level C ABI (include/gsbn_amd.h).
+ *
+ * Per executed step the library launches, per population, one fused
+ * population kernel (one workgroup per hypercolumn) and, per projection, one
+ * row/column kernel over a fixed grid that walks the step's active rows and
+ * active columns; the delay-queue push is a one-workgroup compaction.  No
+ * kernel allocates memory or needs a host round trip: every list length
+ * stays on the device.
+ *
+ * Determinism: every floating-point result is produced by the same sequence
+ * of IEEE operations as the reference's sequential CPU loops
+ * (Pop.cpp:229-323, Proj.cpp:209-536).  In particular the support sum epsc is
+ * accumulated in ascending row order (Proj.cpp:688-690) by the owning
+ * hypercolumn's workgroup at the start of the next population update, the
+ * WTA normaliser is summed in ascending unit order (Pop.cpp:282-292), and
+ * exp/log come from detmath.h.  The file must be compiled with
+ * -ffp-contract=off.
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "../../include/gsbn_amd.h"
+#include "detmath.h"
+#include "gsbn_core.h"
+#include "philox.h"
+
+static_assert(sizeof(gsbn_syn_rec) == 32, "synapse record must be 32 bytes");
+static_assert(sizeof(gsbn_syn_t) == sizeof(gsbn_syn_rec), "record layouts differ");
+static_assert(sizeof(gsbn_row_rec) == 16, "row record must be 16 bytes");
+
+#define POP_MAX_THREADS 1024
+#define RC_THREADS 256
+#define ALL_THREADS 256
+#define ALL_ROWS 16
+
+namespace {
+
+inline int ret(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
+
+inline int launch_status() {
+  hipError_t e = hipGetLastError();
+  return ret(e);
+}
+
+/* exclusive prefix sum across the workgroup (blockDim multiple of 64);
+ * sh needs blockDim/64 + 1 ints */
+__device__ int block_excl_scan(int v, int* sh, int* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int w = 0; w < nw; ++w) {
+      int t = sh[w];
+      sh[w] = acc;
+      acc += t;
+    }
+    sh[nw] = acc;
+  }
+  __syncthreads();
+  int res = sh[wid] + x - v;
+  *total = sh[nw];
+  __syncthreads();
+  return res;
+}
+
+/* epsc of unit (h,j) for incoming projection m after adding the pending row
+ * contributions of the previous step, in ascending row order */
+__device__ float finalized_epsc(const gsbn_pop_args& a, const gsbn_pop_proj& p, int m, int h,
+                                int j, int cnt) {
+  const int64_t N = (int64_t)a.H * a.M;
+  float e = a.epsc[m * N + (int64_t)h * a.M + j];
+  const float* c = p.contrib + ((int64_t)h * p.C) * a.M + j;
+  for (int k = 0; k < cnt; ++k) {
+    float v = c[(int64_t)k * a.M];
+    if (gsbn_fbits(v) != GSBN_NO_CONTRIB_BITS) e = e + v;
+  }
+  return e;
+}
+
+/* support integration of one unit (update_sup_kernel_1_cpu, Pop.cpp:229-261);
+ * returns the new dsup and the draws */
+__device__ float unit_dsup(const gsbn_pop_args& a, int h, int j, const float* efin, float* u,
+                           float* xi) {
+  const int64_t N = (int64_t)a.H * a.M;
+  const int idx = h * a.M + j;
+  float z;
+  gsbn_draw_unit(a.seed, (uint32_t)a.pop_id, (uint32_t)a.simstep, (uint32_t)idx, u, &z);
+  *xi = a.snoise * z;
+  float wsup = 0.0f;
+  for (int m = 0; m < a.nproj_in; ++m) wsup += a.bj[m * N + idx] + efin[m];
+  float sup = a.lgbias + a.igain * a.lginp[idx] + *xi;
+  sup += (a.wgain * a.wmask[h]) * wsup;
+  float dsup = a.dsup[idx];
+  return dsup + (sup - dsup) * a.taumdt;
+}
+
+/* dsup of the population's first unit for the CPU-quirk WTA maximum */
+__global__ void pop_dsup0_kernel(gsbn_pop_args a) {
+  if (threadIdx.x != 0) return;
+  float efin[GSBN_MAX_PROJ_IN];
+  for (int m = 0; m < a.nproj_in; ++m) efin[m] = finalized_epsc(a, a.projs[m], m, 0, 0, a.projs[m].pend_cnt[0]);
+  float u, xi;
+  a.scratch_dsup0[0] = unit_dsup(a, 0, 0, efin, &u, &xi);
+}
+
+__global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
+  extern __shared__ uint32_t sh_bitmap[];
+  __shared__ float sh_f[POP_MAX_THREADS];
+  __shared__ int sh_ext[POP_MAX_THREADS];
+  __shared__ int sh_scan[POP_MAX_THREADS / 64 + 1];
+  __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
+  __shared__ float sh_vsum;
+
+  const int h = blockIdx.x;
+  const int j = threadIdx.x;
+  const int M = a.M;
+  const bool live = j < M;
+  const int idx = h * M + j;
+  const int64_t N = (int64_t)a.H * M;
+
+  if (j < a.nproj_in) sh_pend[j] = a.projs[j].pend_cnt[h];
+  __syncthreads();
+
+  /* 1. pending support contributions of the previous step's row updates */
+  float efin[GSBN_MAX_PROJ_IN];
+  for (int m = 0; m < a.nproj_in; ++m) {
+    efin[m] = live ? finalized_epsc(a, a.projs[m], m, h, j, sh_pend[m]) : 0.0f;
+    if (live) a.epsc[m * N + idx] = efin[m];
+  }
+
+  /* 2. random draws, support, dsup */
+  float u = 0.0f, xi = 0.0f, dsup = 0.0f;
+  if (live) {
+    dsup = unit_dsup(a, h, j, efin, &u, &xi);
+    a.dsup[idx] = dsup;
+    a.rnd_u[idx] = u;
+    a.rnd_n[idx] = xi;
+    sh_f[j] = dsup;
+  }
+  __syncthreads();
+
+  /* 3. WTA maximum (update_sup_kernel_2_cpu, Pop.cpp:270-279); tree over
+   * the next power of two >= M, reading only indices < M */
+  int p2 = 1;
+  while (p2 < M) p2 <<= 1;
+  for (int s = p2 / 2; s > 0; s >>= 1) {
+    if (j < s && j + s < M) {
+      float o = sh_f[j + s];
+      if (o > sh_f[j]) sh_f[j] = o;
+    }
+    __syncthreads();
+  }
+  float maxdsup = sh_f[0];
+  if (a.wta_pop_max) {
+    float d0 = a.scratch_dsup0[0];
+    float mx = d0;
+    if (maxdsup > mx) mx = maxdsup;
+    maxdsup = mx;
+  }
+  __syncthreads();
+
+  /* soft-max (Pop.cpp:280-302) */
+  const float maxact = gsbn_expf(a.wtagain * maxdsup);
+  float act = 0.0f;
+  if (live) {
+    act = gsbn_expf(a.wtagain * (dsup - maxdsup));
+    if (maxact < 1.0f) act *= maxact;
+    sh_f[j] = act;
+  }
+  __syncthreads();
+  if (j == 0) {
+    float vsum = 0.0f;
+    for (int m = 0; m < M; ++m) vsum += sh_f[m];
+    sh_vsum = vsum;
+  }
+  __syncthreads();
+  const float vsum = sh_vsum;
+  if (live && vsum > 1.0f) act /= vsum;
+
+  /* 4. spikes and counters (Pop.cpp:305-323); external override (Pop.cpp:389-409) */
+  int spk = 0;
+  if (live) {
+    a.act[idx] = act;
+    spk = (u < act * a.maxfqdt) ? 1 : 0;
+    a.counter[idx] += spk;
+  }
+  if (a.ext_off) {
+    sh_ext[j] = 0;
+    __syncthreads();
+    if (a.simstep >= 0 && a.simstep < a.ext_nsteps) {
+      const int b = a.ext_off[a.simstep], e = a.ext_off[a.simstep + 1];
+      for (int t = b + j; t < e; t += blockDim.x) {
+        int unit = a.ext_units[t];
+        if (unit >= h * M && unit < h * M + M) sh_ext[unit - h * M] = 1;
+      }
+    }
+    __syncthreads();
+    if (live) spk = sh_ext[j];
+  }
+  if (live) a.spike[idx] = (int8_t)spk;
+
+  /* 5. spiking units of this hypercolumn, ascending (.ssj, Proj.cpp:632-643) */
+  int spk_tot;
+  {
+    int pos = block_excl_scan(live ? spk : 0, sh_scan, &spk_tot);
+    if (live && spk) a.spk_list[h * M + pos] = j;
+    if (j == 0) a.spk_cnt[h] = spk_tot;
+  }
+
+  /* 6. postsynaptic traces and bias for every incoming projection
+   * (update_jxx_kernel_cpu, Proj.cpp:310-363) */
+  if (a.do_jxx && live) {
+    for (int m = 0; m < a.nproj_in; ++m) {
+      const gsbn_pop_proj& p = a.projs[m];
+      const float kp = p.taupdt * a.prn;
+      float pj = p.pj[idx], ej = p.ej[idx], zj = p.zj[idx];
+      a.epsc[m * N + idx] = efin[m] * (1 - p.kepsc);
+      pj += (ej - pj) * kp;
+      ej += (zj - ej) * p.ke;
+      zj *= (1 - p.kzj);
+      if (spk > 0) zj += p.kftj;
+      if (kp != 0.0f) a.bj[m * N + idx] = p.bgain * gsbn_logf(pj + p.eps);
+      p.pj[idx] = pj;
+      p.ej[idx] = ej;
+      p.zj[idx] = zj;
+    }
+  }
+
+  /* 7. active rows of this step from the delay queue (Proj.cpp:621-630,
+   * 645-664): row r = h*C+k is active at exec x iff its source unit ii was
+   * pushed at the queue update of exec x - di - 1 */
+  for (int m = 0; m < a.nproj_in; ++m) {
+    const gsbn_pop_proj& p = a.projs[m];
+    const int C = p.C;
+    const int W = (C + 31) >> 5;
+    for (int w = j; w < W; w += blockDim.x) sh_bitmap[w] = 0u;
+    __syncthreads();
+    for (int d = p.dmin; d <= p.dmax; ++d) {
+      const int e = a.exec - d - 1;
+      if (e < 0) continue;
+      const int slot = e % GSBN_HIST;
+      const int cnt = p.hist_cnt[slot];
+      const int32_t* lst = p.hist_list + (int64_t)slot * p.Nsrc;
+      const int32_t* off = p.inv_off + (int64_t)h * (p.Nsrc + 1);
+      for (int t = j; t < cnt; t += blockDim.x) {
+        const int n = lst[t];
+        for (int q = off[n]; q < off[n + 1]; ++q) {
+          const int k = p.inv_slot[q];
+          if (p.di[(int64_t)h * C + k] == d) atomicOr(&sh_bitmap[k >> 5], 1u << (k & 31));
+        }
+      }
+    }
+    __syncthreads();
+    const int per = (W + blockDim.x - 1) / blockDim.x;
+    const int w0 = j * per;
+    const int w1 = (w0 + per < W) ? w0 + per : W;
+    int c = 0;
+    for (int w = w0; w < w1; ++w) c += __popc(sh_bitmap[w]);
+    int tot;
+    int pos = block_excl_scan(c, sh_scan, &tot);
+    for (int w = w0; w < w1; ++w) {
+      uint32_t bits = sh_bitmap[w];
+      while (bits) {
+        const int b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const int k = (w << 5) + b;
+        p.rows_list[(int64_t)h * C + pos] = k;
+        p.act_stamp[(int64_t)h * C + k] = a.exec;
+        ++pos;
+      }
+    }
+    if (j == 0) {
+      p.rows_cnt[h] = tot;
+      p.pend_cnt[h] = tot;
+      if (a.work) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.work[2 * m]),
+                  (unsigned long long)tot * (unsigned long long)M);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.work[2 * m + 1]),
+                  (unsigned long long)spk_tot * (unsigned long long)C);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_args a) {
+  __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
+  const int h = blockIdx.x, j = threadIdx.x;
+  const int64_t N = (int64_t)a.H * a.M;
+  if (j < a.nproj_in) sh_pend[j] = a.projs[j].pend_cnt[h];
+  __syncthreads();
+  if (j < a.M) {
+    for (int m = 0; m < a.nproj_in; ++m)
+      a.epsc[m * N + (int64_t)h * a.M + j] = finalized_epsc(a, a.projs[m], m, h, j, sh_pend[m]);
+  }
+  __syncthreads();
+  if (j < a.nproj_in) a.projs[j].pend_cnt[h] = 0;
+}
+
+/* largest h in [0, n-1] with off[h] <= g (off non-decreasing, off[0] = 0) */
+__device__ inline int find_owner(const int* off, int n, int g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(RC_THREADS) rowcol_kernel(gsbn_rowcol_args a) {
+  extern __shared__ int sh_off[]; /* row_off[H+1], col_off[H+1] */
+  __shared__ int sh_scan[RC_THREADS / 64 + 1];
+  __shared__ float sh_pi[RC_THREADS];
+  __shared__ int sh_h[RC_THREADS];
+  __shared__ int sh_a[RC_THREADS];
+  __shared__ int sh_r[RC_THREADS];
+
+  const int H = a.H, M = a.M, C = a.C;
+  const int tid = threadIdx.x;
+  int* row_off = sh_off;
+  int* col_off = sh_off + (H + 1);
+  const int nblk = (C + RC_THREADS - 1) / RC_THREADS;
+
+  int run_r = 0, run_c = 0;
+  for (int base = 0; base < H; base += RC_THREADS) {
+    const int hh = base + tid;
+    const int vr = hh < H ? a.rows_cnt[hh] : 0;
+    const int vc = hh < H ? a.spk_cnt[hh] * nblk : 0;
+    int tr, tc;
+    const int er = block_excl_scan(vr, sh_scan, &tr);
+    const int ec = block_excl_scan(vc, sh_scan, &tc);
+    if (hh < H) {
+      row_off[hh] = run_r + er;
+      col_off[hh] = run_c + ec;
+    }
+    run_r += tr;
+    run_c += tc;
+  }
+  if (tid == 0) {
+    row_off[H] = run_r;
+    col_off[H] = run_c;
+  }
+  __syncthreads();
+
+  const int A = run_r, Q = run_c;
+  const int rpt = M >= RC_THREADS ? 1 : RC_THREADS / M;
+  const int ntr = (A + rpt - 1) / rpt;
+  const int ntask = ntr + Q;
+  const int t = a.simstep;
+
+  for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
+    if (task < ntr) {
+      const int g0 = task * rpt;
+      const int nr = (A - g0 < rpt) ? A - g0 : rpt;
+      if (tid < nr) {
+        const int g = g0 + tid;
+        const int h = find_owner(row_off, H, g);
+        const int aa = g - row_off[h];
+        const int k = a.rows_list[(int64_t)h * C + aa];
+        const int r = h * C + k;
+        gsbn_row_rec st = a.rowst[r];
+        const int D = t - st.ti;
+        if (D <= 0) {
+          st.zi += a.kfti;
+          st.ti = t;
+        } else {
+          const float* c = gsbn_coef(a.table, a.table_len, D);
+          gsbn_row_catchup(c, &st.pi, &st.ei, &st.zi);
+          st.zi = st.zi + a.kfti;
+          st.ti = t;
+        }
+        a.rowst[r] = st;
+        sh_pi[tid] = st.pi;
+        sh_h[tid] = h;
+        sh_a[tid] = aa;
+        sh_r[tid] = r;
+      }
+      __syncthreads();
+      for (int f = tid; f < nr * M; f += RC_THREADS) {
+        const int rr = f / M;
+        const int j = f - rr * M;
+        const int h = sh_h[rr];
+        const int64_t sidx = (int64_t)sh_r[rr] * M + j;
+        gsbn_syn_rec s = a.syn[sidx];
+        float contrib;
+        const int D = t - s.tij;
+        if (D <= 0) {
+          s.zi2 += a.kfti;
+          s.tij = t;
+          contrib = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
+        } else {
+          const float* c = gsbn_coef(a.table, a.table_len, D);
+          gsbn_syn_catchup(c, &s.pij, &s.eij, &s.zi2, &s.zj2);
+          s.zi2 = s.zi2 + a.kfti;
+          s.tij = t;
+          float w;
+          if (a.kp != 0.0f) {
+            w = gsbn_wij(s.pij, sh_pi[rr], a.pj[h * M + j], a.wgain, a.eps, a.eps2);
+            s.wij = w;
+          } else {
+            w = s.wij;
+          }
+          contrib = w;
+        }
+        /* column increment of a spiking postsynaptic unit after the row
+         * update (the reference's col pass sees D == 0 here) */
+        if (a.sj[h * M + j] > 0) s.zj2 += a.kftj;
+        a.syn[sidx] = s;
+        a.contrib[((int64_t)h * C + sh_a[rr]) * M + j] = contrib;
+      }
+      __syncthreads();
+    } else {
+      const int q = task - ntr;
+      const int h = find_owner(col_off, H, q);
+      const int local = q - col_off[h];
+      const int s_i = local / nblk;
+      const int b = local - s_i * nblk;
+      const int j = a.spk_list[h * M + s_i];
+      const int k = b * RC_THREADS + tid;
+      if (k < C) {
+        const int r = h * C + k;
+        if (a.ii[r] >= 0 && a.act_stamp[r] != a.exec) {
+          gsbn_syn_rec* sp = a.syn + (int64_t)r * M + j;
+          float4 v = *reinterpret_cast<float4*>(sp);
+          int tij = sp->tij;
+          float pij = v.x, eij = v.y, zi2 = v.z, zj2 = v.w;
+          const int D = t - tij;
+          if (D <= 0) {
+            zj2 += a.kftj;
+          } else {
+            const float* c = gsbn_coef(a.table, a.table_len, D);
+            gsbn_syn_catchup(c, &pij, &eij, &zi2, &zj2);
+            zj2 = zj2 + a.kftj;
+          }
+          *reinterpret_cast<float4*>(sp) = make_float4(pij, eij, zi2, zj2);
+          sp->tij = t;
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
+  __shared__ float sh_pi[ALL_ROWS];
+  const int64_t R = (int64_t)a.H * a.C;
+  const int M = a.M;
+  const int tid = threadIdx.x;
+  const int t = a.target;
+  const int64_t nchunks = (R + ALL_ROWS - 1) / ALL_ROWS;
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int64_t r0 = ch * ALL_ROWS;
+    const int nr = (R - r0 < ALL_ROWS) ? (int)(R - r0) : ALL_ROWS;
+    if (tid < nr) {
+      const int64_t r = r0 + tid;
+      gsbn_row_rec st = a.rowst[r];
+      const int D = t - st.ti;
+      if (D > 0) {
+        const float* c = gsbn_coef(a.table, a.table_len, D);
+        gsbn_row_catchup(c, &st.pi, &st.ei, &st.zi);
+        st.ti = t;
+        a.rowst[r] = st;
+      }
+      sh_pi[tid] = st.pi;
+    }
+    __syncthreads();
+    for (int f = tid; f < nr * M; f += ALL_THREADS) {
+      const int rr = f / M;
+      const int j = f - rr * M;
+      const int64_t r = r0 + rr;
+      gsbn_syn_rec* sp = a.syn + r * M + j;
+      gsbn_syn_rec s = *sp;
+      const int D = t - s.tij;
+      if (D <= 0) {
+        sp->tij = t;
+      } else {
+        const float* c = gsbn_coef(a.table, a.table_len, D);
+        gsbn_syn_catchup(c, &s.pij, &s.eij, &s.zi2, &s.zj2);
+        s.tij = t;
+        if (a.kp != 0.0f) {
+          const int h = (int)(r / a.C);
+          s.wij = gsbn_wij(s.pij, sh_pi[rr], a.pj[h * M + j], a.wgain, a.eps, a.eps2);
+        }
+        *sp = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void jxx_kernel(gsbn_jxx_args a) {
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < a.N; idx += gridDim.x * blockDim.x) {
+    float pj = a.pj[idx], ej = a.ej[idx], zj = a.zj[idx];
+    const int sj = a.sj[idx];
+    a.epsc[idx] *= (1 - a.kepsc);
+    pj += (ej - pj) * a.kp;
+    ej += (zj - ej) * a.ke;
+    zj *= (1 - a.kzj);
+    if (sj > 0) zj += a.kftj;
+    if (a.kp != 0.0f) a.bj[idx] = a.bgain * gsbn_logf(pj + a.eps);
+    a.pj[idx] = pj;
+    a.ej[idx] = ej;
+    a.zj[idx] = zj;
+  }
+}
+
+__global__ void __launch_bounds__(1024) que_kernel(gsbn_que_args a) {
+  __shared__ int sh_scan[1024 / 64 + 1];
+  const int slot = a.exec % GSBN_HIST;
+  int32_t* out = a.hist_list + (int64_t)slot * a.Nsrc;
+  int running = 0;
+  for (int base = 0; base < a.Nsrc; base += blockDim.x) {
+    const int n = base + threadIdx.x;
+    const int flag = (n < a.Nsrc && a.si[n] > 0) ? 1 : 0;
+    int tot;
+    const int pos = block_excl_scan(flag, sh_scan, &tot);
+    if (flag) out[running + pos] = n;
+    running += tot;
+  }
+  if (threadIdx.x == 0) a.hist_cnt[slot] = running;
+}
+
+__global__ void check_count_kernel(int n, const int8_t* spike, int32_t* count) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    count[i] += (spike[i] > 0) ? 1 : 0;
+}
+
+__global__ void check_argmax_kernel(int H, int M, int32_t* count, int32_t* out_idx,
+                                    int32_t* out_cnt) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= H) return;
+  int mi = -1, mc = 0;
+  for (int j = 0; j < M; ++j) {
+    const int v = count[h * M + j];
+    if (v > mc) {
+      mi = j;
+      mc = v;
+    }
+    count[h * M + j] = 0;
+  }
+  out_idx[h] = mi;
+  out_cnt[h] = mc;
+}
+
+__global__ void syn_field_kernel(gsbn_syn_rec* syn, int64_t n, int field, uint32_t* buf,
+                                 int to_records) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t* rec = reinterpret_cast<uint32_t*>(syn + i);
+    const int off = field == 5 ? 5 : field;
+    if (to_records)
+      rec[off] = buf[i];
+    else
+      buf[i] = rec[off];
+  }
+}
+
+__global__ void row_field_kernel(gsbn_row_rec* row, int64_t n, int field, uint32_t* buf,
+                                 int to_records) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t* rec = reinterpret_cast<uint32_t*>(row + i);
+    if (to_records)
+      rec[field] = buf[i];
+    else
+      buf[i] = rec[field];
+  }
+}
+
+__global__ void syn_fill_kernel(gsbn_syn_rec* syn, int64_t n, gsbn_syn_rec v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    syn[i] = v;
+}
+
+__global__ void row_fill_kernel(gsbn_row_rec* row, int64_t n, gsbn_row_rec v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    row[i] = v;
+}
+
+__global__ void i32_fill_kernel(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void device_math_kernel(int op, int64_t n, const uint32_t* in, uint32_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (op == 0) {
+      out[i] = gsbn_fbits(gsbn_expf(gsbn_bitsf(in[i])));
+    } else if (op == 1) {
+      out[i] = gsbn_fbits(gsbn_logf(gsbn_bitsf(in[i])));
+    } else if (op == 2) {
+      float u, z;
+      const uint64_t seed = (uint64_t)in[4 * i] | ((uint64_t)in[4 * i + 1] << 32);
+      gsbn_draw_unit(seed, 0u, in[4 * i + 2], in[4 * i + 3], &u, &z);
+      out[2 * i] = gsbn_fbits(u);
+      out[2 * i + 1] = gsbn_fbits(z);
+    } else if (op == 3) {
+      out[i] = gsbn_fbits(gsbn_bitsf(in[2 * i]) / gsbn_bitsf(in[2 * i + 1]));
+    } else if (op == 4) {
+      out[i] = gsbn_fbits(gsbn_wij(gsbn_bitsf(in[3 * i]), gsbn_bitsf(in[3 * i + 1]),
+                                   gsbn_bitsf(in[3 * i + 2]), 1.0f, 1e-3f, 1e-6f));
+    }
+  }
+}
+
+int grid_for(int64_t n, int threads, int cap) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsbn_pop_update(const gsbn_pop_args* a, void* stream) {
+  if (!a || a->H <= 0 || a->M <= 0 || a->M > POP_MAX_THREADS || a->nproj_in < 0 ||
+      a->nproj_in > GSBN_MAX_PROJ_IN || (a->nproj_in > 0 && !a->projs) || a->max_slots < 0)
+    return GSBN_ERR_ARG;
+  if (a->wta_pop_max && !a->scratch_dsup0) return GSBN_ERR_ARG;
+  const size_t lds = (size_t)((a->max_slots + 31) / 32) * sizeof(uint32_t);
+  if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
+  hipStream_t s = (hipStream_t)stream;
+  const int threads = ((a->M + 63) / 64) * 64;
+  if (a->wta_pop_max) {
+    hipLaunchKernelGGL(pop_dsup0_kernel, dim3(1), dim3(64), 0, s, *a);
+    int rc = launch_status();
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(pop_kernel, dim3(a->H), dim3(threads), lds, s, *a);
+  return launch_status();
+}
+
+int gsbn_pop_flush(const gsbn_pop_args* a, void* stream) {
+  if (!a || a->H <= 0 || a->M <= 0 || a->M > POP_MAX_THREADS || a->nproj_in < 0 ||
+      a->nproj_in > GSBN_MAX_PROJ_IN || (a->nproj_in > 0 && !a->projs))
+    return GSBN_ERR_ARG;
+  const int threads = ((a->M + 63) / 64) * 64;
+  hipLaunchKernelGGL(pop_flush_kernel, dim3(a->H), dim3(threads), 0, (hipStream_t)stream, *a);
+  return launch_status();
+}
+
+int gsbn_proj_all(const gsbn_all_args* a, void* stream) {
+  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2) return GSBN_ERR_ARG;
+  const int64_t R = (int64_t)a->H * a->C;
+  if (R == 0) return 0;
+  const int grid = grid_for((R + ALL_ROWS - 1) / ALL_ROWS, 1, 8192);
+  hipLaunchKernelGGL(all_kernel, dim3(grid), dim3(ALL_THREADS), 0, (hipStream_t)stream, *a);
+  return launch_status();
+}
+
+int gsbn_proj_jxx(const gsbn_jxx_args* a, void* stream) {
+  if (!a || a->N < 0) return GSBN_ERR_ARG;
+  if (a->N == 0) return 0;
+  hipLaunchKernelGGL(jxx_kernel, dim3(grid_for(a->N, 256, 4096)), dim3(256), 0,
+                     (hipStream_t)stream, *a);
+  return launch_status();
+}
+
+int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream) {
+  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2) return GSBN_ERR_ARG;
+  const size_t lds = (size_t)2 * (a->H + 1) * sizeof(int);
+  if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
+  const int grid = a->grid > 0 ? a->grid : 1024;
+  hipLaunchKernelGGL(rowcol_kernel, dim3(grid), dim3(RC_THREADS), lds, (hipStream_t)stream, *a);
+  return launch_status();
+}
+
+int gsbn_proj_que(const gsbn_que_args* a, void* stream) {
+  if (!a || a->Nsrc < 0 || a->exec < 0) return GSBN_ERR_ARG;
+  hipLaunchKernelGGL(que_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, *a);
+  return launch_status();
+}
+
+int gsbn_check_count(int32_t n, const int8_t* spike, int32_t* count, void* stream) {
+  if (n < 0 || (n && (!spike || !count))) return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(check_count_kernel, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
+                     (hipStream_t)stream, n, spike, count);
+  return launch_status();
+}
+
+int gsbn_check_argmax(int32_t H, int32_t M, int32_t* count, int32_t* out_idx, int32_t* out_cnt,
+                      void* stream) {
+  if (H <= 0 || M <= 0 || !count || !out_idx || !out_cnt) return GSBN_ERR_ARG;
+  hipLaunchKernelGGL(check_argmax_kernel, dim3((H + 63) / 64), dim3(64), 0, (hipStream_t)stream,
+                     H, M, count, out_idx, out_cnt);
+  return launch_status();
+}
+
+int gsbn_syn_field(gsbn_syn_rec* syn, int64_t n, int field, void* buf, int to_records,
+                   void* stream) {
+  if (n < 0 || field < 0 || field > 5 || (n && (!syn || !buf))) return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(syn_field_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
+                     (hipStream_t)stream, syn, n, field, (uint32_t*)buf, to_records);
+  return launch_status();
+}
+
+int gsbn_row_field(gsbn_row_rec* row, int64_t n, int field, void* buf, int to_records,
+                   void* stream) {
+  if (n < 0 || field < 0 || field > 3 || (n && (!row || !buf))) return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(row_field_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
+                     (hipStream_t)stream, row, n, field, (uint32_t*)buf, to_records);
+  return launch_status();
+}
+
+int gsbn_syn_fill(gsbn_syn_rec* syn, int64_t n, gsbn_syn_rec v, void* stream) {
+  if (n < 0 || (n && !syn)) return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(syn_fill_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
+                     (hipStream_t)stream, syn, n, v);
+  return launch_status();
+}
+
+int gsbn_row_fill(gsbn_row_rec* row, int64_t n, gsbn_row_rec v, void* stream) {
+  if (n < 0 || (n && !row)) return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(row_fill_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
+                     (hipStream_t)stream, row, n, v);
+  return launch_status();
+}
+
+int gsbn_i32_fill(int32_t* p, int64_t n, int32_t v, void* stream) {
+  if (n < 0 || (n && !p)) return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(i32_fill_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
+                     (hipStream_t)stream, p, n, v);
+  return launch_status();
+}
+
+int gsbn_device_math(int op, int64_t n, const void* in_host, void* out_host) {
+  if (op < 0 || op > 4 || n < 0) return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  const int in_w = op == 2 ? 4 : op == 3 ? 2 : op == 4 ? 3 : 1;
+  const int out_w = op == 2 ? 2 : 1;
+  uint32_t *din = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&din, (size_t)n * in_w * 4);
+  if (e != hipSuccess) return ret(e);
+  e = hipMalloc(&dout, (size_t)n * out_w * 4);
+  if (e != hipSuccess) {
+    (void)hipFree(din);
+    return ret(e);
+  }
+  e = hipMemcpy(din, in_host, (size_t)n * in_w * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(device_math_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, 0, op, n,
+                       din, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out_host, dout, (size_t)n * out_w * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return ret(e);
+}
+
+}  // extern "C"

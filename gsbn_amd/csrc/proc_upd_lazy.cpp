@@ -1,0 +1,949 @@
+/*
+ * proc_upd_lazy.cpp -- the ProcUpdLazy procedure (ProcUpdLazy.cpp:8-180) and
+ * its Pop (Pop.cpp:8-409) and Proj (Proj.cpp:6-760) state owners, driving
+ * the kernels of kernels.hip through the kernel-level C ABI.
+ *
+ * Database contract: the same vector names, element types and row-major
+ * [hcu][slot][mcu] semantics as the reference (Pop.cpp:47-57,
+ * Proj.cpp:64-86).  The per-synapse and per-row traces live in record arrays
+ * on the device; their named vectors (pij_<id>, ..., pi_<id>, ...) are views
+ * that gather a field on host reads and scatter it back before the next step
+ * after host writes.  The delay queue is kept as a ring of the source units
+ * pushed at each executed step; qi_<id>, .siq_<id> and .ssi_<id> are
+ * materialised from it on host reads.
+ *
+ * Differences from the reference, by design: update_cpu() raises (this
+ * build's step runs on the GPU only; there is no silent CPU fallback),
+ * init_copy() raises (snapshot resume is outside this build), random draws
+ * are Philox streams keyed by (seed, population, step, unit), and the
+ * closed-form catch-up uses the FP64-built coefficient table.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "../../include/gsbn_amd.h"
+#include "coeff.h"
+#include "detmath.h"
+#include "framework.h"
+#include "gsbn_core.h"
+
+namespace gsbn {
+namespace proc_upd_lazy {
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T* h, size_t count) {
+    if (count > n) alloc(count);
+    if (count)
+      hip_check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, active_stream()),
+                "H2D");
+    hip_check(hipStreamSynchronize(active_stream()), "sync");
+  }
+  void download(T* h, size_t count) const {
+    hip_check(hipStreamSynchronize(active_stream()), "sync");
+    if (count) hip_check(hipMemcpy(h, p, count * sizeof(T), hipMemcpyDeviceToHost), "D2H");
+  }
+};
+
+class Proj;
+
+class Pop {
+ public:
+  void init_new(ProcParam proc_param, PopParam pop_param, Database& db, vector<Pop*>* list_pop,
+                int* hcu_cnt, int* mcu_cnt);
+  void update(int simstep, int exec, float prn, int lginp_idx, int wmask_idx, bool do_jxx,
+              uint64_t seed, int wta_pop_max);
+  void flush();
+  gsbn_pop_args args(int simstep, int exec, float prn, int lginp_idx, int wmask_idx, bool do_jxx,
+                     uint64_t seed, int wta_pop_max);
+
+  int _id = 0, _rank = 0, _dim_proj = 0, _dim_hcu = 0, _dim_mcu = 0, _hcu_start = 0, _mcu_start = 0;
+  float _taumdt = 0, _wtagain = 0, _maxfqdt = 0, _igain = 0, _wgain = 0, _lgbias = 0, _snoise = 0;
+  SyncVector<float>*_epsc = nullptr, *_bj = nullptr, *_dsup = nullptr, *_act = nullptr, *_ada = nullptr;
+  SyncVector<float>*_rnd_uniform01 = nullptr, *_rnd_normal = nullptr, *_wmask = nullptr, *_lginp = nullptr;
+  SyncVector<int8_t>* _spike = nullptr;
+  SyncVector<int32_t>* _counter = nullptr;
+  vector<Proj*> _in; /* incoming projections in proj_in_pop order */
+  DevBuf<int32_t> spk_list, spk_cnt, ext_off, ext_units;
+  DevBuf<float> scratch;
+  DevBuf<gsbn_pop_proj> projs_dev;
+  DevBuf<int64_t> work;
+  vector<gsbn_pop_proj> projs_host;
+  int ext_nsteps = 0;
+  bool has_ext = false;
+  bool flush_pending = false;
+};
+
+struct CoefTable {
+  DevBuf<float> dev;
+  int64_t len = 0;
+  bool saturated = false;
+};
+
+class Proj {
+ public:
+  void init_new(ProcParam proc_param, ProjParam proj_param, Database& db, vector<Proj*>* list_proj,
+                vector<Pop*>* list_pop);
+  void prepare(int simstep);
+  void update_all(int simstep, float old_prn);
+  void update_jxx(float prn);
+  void update_rowcol(int simstep, int exec, float prn);
+  void update_que(int exec);
+  void mark_views_device();
+  gsbn_pop_proj pop_desc();
+  const CoefTable& table(float kp, int64_t need);
+  void rebuild_fanout();
+
+  int _id = 0, _rank = 0, _proj_in_pop = 0, _dim_conn = 0, _dim_hcu = 0, _dim_mcu = 0, _nsrc = 0;
+  float _taupdt = 0, _tauedt = 0, _tauzidt = 0, _tauzjdt = 0, _tauepscdt = 0, _eps = 0, _eps2 = 0;
+  float _kfti = 0, _kftj = 0, _wgain = 0, _bgain = 0;
+  Pop* _ptr_src_pop = nullptr;
+  Pop* _ptr_dest_pop = nullptr;
+  SyncVector<int32_t>*_ii = nullptr, *_qi = nullptr, *_di = nullptr, *_ssi = nullptr, *_ssj = nullptr,
+                     *_ti = nullptr, *_tij = nullptr;
+  SyncVector<int8_t>*_siq = nullptr, *_si = nullptr;
+  SyncVector<float>*_pi = nullptr, *_ei = nullptr, *_zi = nullptr, *_pj = nullptr, *_ej = nullptr,
+                   *_zj = nullptr, *_pij = nullptr, *_eij = nullptr, *_zi2 = nullptr, *_zj2 = nullptr,
+                   *_wij = nullptr;
+  vector<VecBase*> views;
+  /* device state */
+  DevBuf<gsbn_syn_rec> syn;
+  DevBuf<gsbn_row_rec> rowst;
+  DevBuf<float> contrib;
+  DevBuf<int32_t> rows_list, rows_cnt, pend_cnt, act_stamp, hist_list, hist_cnt, inv_off, inv_slot;
+  std::map<uint32_t, std::unique_ptr<CoefTable>> tables;
+  uint64_t ii_version = ~0ull, di_version = ~0ull;
+  int dmin = 1, dmax = 0;
+  int last_exec = -1;
+  int sim_end = 0;
+  std::vector<std::unique_ptr<VecView>> view_objs;
+};
+
+/* ---- views ---- */
+
+/* the host buffer of a vector whose to_host() is in progress */
+static void* raw_data(VecBase* v) {
+  switch (v->type) {
+    case DT_F32: return static_cast<SyncVector<float>*>(v)->raw_host().data();
+    case DT_I32: return static_cast<SyncVector<int32_t>*>(v)->raw_host().data();
+    case DT_I8: return static_cast<SyncVector<int8_t>*>(v)->raw_host().data();
+    default: throw Error(-4, "unexpected view type");
+  }
+}
+
+struct SynField : VecView {
+  Proj* p;
+  int field;
+  SynField(Proj* q, int f) : p(q), field(f) {}
+  void pull(VecBase* v) override {
+    const size_t n = p->syn.n;
+    DevBuf<uint32_t> tmp;
+    tmp.alloc(n);
+    op_check(gsbn_syn_field(p->syn.p, (int64_t)n, field, tmp.p, 0, active_stream()), "gsbn_syn_field");
+    tmp.download((uint32_t*)raw_data(v), n);
+  }
+  void push(VecBase* v) override {
+    const size_t n = p->syn.n;
+    DevBuf<uint32_t> tmp;
+    tmp.upload((const uint32_t*)raw_data(v), n);
+    op_check(gsbn_syn_field(p->syn.p, (int64_t)n, field, tmp.p, 1, active_stream()), "gsbn_syn_field");
+    hip_check(hipStreamSynchronize(active_stream()), "sync");
+  }
+};
+
+struct RowField : VecView {
+  Proj* p;
+  int field;
+  RowField(Proj* q, int f) : p(q), field(f) {}
+  void pull(VecBase* v) override {
+    const size_t n = p->rowst.n;
+    DevBuf<uint32_t> tmp;
+    tmp.alloc(n);
+    op_check(gsbn_row_field(p->rowst.p, (int64_t)n, field, tmp.p, 0, active_stream()), "gsbn_row_field");
+    tmp.download((uint32_t*)raw_data(v), n);
+  }
+  void push(VecBase* v) override {
+    const size_t n = p->rowst.n;
+    DevBuf<uint32_t> tmp;
+    tmp.upload((const uint32_t*)raw_data(v), n);
+    op_check(gsbn_row_field(p->rowst.p, (int64_t)n, field, tmp.p, 1, active_stream()), "gsbn_row_field");
+    hip_check(hipStreamSynchronize(active_stream()), "sync");
+  }
+};
+
+/* views materialised from the queue ring / step lists; read-only */
+struct QueueView : VecView {
+  Proj* p;
+  int what; /* 0 qi, 1 siq, 2 ssi, 3 ssj */
+  QueueView(Proj* q, int w) : p(q), what(w) {}
+  void push(VecBase* v) override {
+    throw Error(-5, "vector " + v->name + " is derived from the device spike queue and is read-only");
+  }
+  void pull(VecBase* v) override;
+};
+
+/* bring the host copy of a view up to date without re-entering to_host */
+template <typename T>
+static vector<T>& raw(VecBase* v) {
+  return static_cast<SyncVector<T>*>(v)->raw_host();
+}
+
+void QueueView::pull(VecBase* v) {
+  const int H = p->_dim_hcu, C = p->_dim_conn;
+  const int64_t R = (int64_t)H * C;
+  if (what == 2) {
+    vector<int32_t> cnt(H), lst;
+    p->rows_cnt.download(cnt.data(), H);
+    lst.resize((size_t)R);
+    p->rows_list.download(lst.data(), (size_t)R);
+    auto& out = raw<int32_t>(v);
+    out.clear();
+    for (int h = 0; h < H; ++h)
+      for (int a = 0; a < cnt[h]; ++a) out.push_back(h * C + lst[(size_t)h * C + a]);
+    static_cast<SyncVector<int32_t>*>(v)->set_logical_size(out.size());
+    return;
+  }
+  if (what == 3) {
+    Pop* d = p->_ptr_dest_pop;
+    const int M = d->_dim_mcu;
+    vector<int32_t> cnt(d->_dim_hcu), lst((size_t)d->_dim_hcu * M);
+    d->spk_cnt.download(cnt.data(), cnt.size());
+    d->spk_list.download(lst.data(), lst.size());
+    auto& out = raw<int32_t>(v);
+    out.clear();
+    for (int h = 0; h < d->_dim_hcu; ++h)
+      for (int a = 0; a < cnt[h]; ++a) out.push_back(h * M + lst[(size_t)h * M + a]);
+    static_cast<SyncVector<int32_t>*>(v)->set_logical_size(out.size());
+    return;
+  }
+  /* qi / siq from the ring: que at exec e pushed bit (di-1); after the que
+   * of exec x it sits at bit (di-1)-(x-e); siq(x) = pushed at x-di */
+  const int x = p->last_exec;
+  vector<int32_t> hc(GSBN_HIST);
+  p->hist_cnt.download(hc.data(), GSBN_HIST);
+  const int lo = std::max(0, x - 32);
+  const int span = x - lo + 1;
+  vector<uint8_t> present;
+  if (x >= 0) {
+    present.assign((size_t)span * p->_nsrc, 0);
+    vector<int32_t> row(p->_nsrc);
+    for (int e = lo; e <= x; ++e) {
+      const int slot = e % GSBN_HIST;
+      if (hc[slot] <= 0) continue;
+      hip_check(hipStreamSynchronize(active_stream()), "sync");
+      hip_check(hipMemcpy(row.data(), p->hist_list.p + (int64_t)slot * p->_nsrc, hc[slot] * 4,
+                          hipMemcpyDeviceToHost), "D2H");
+      for (int i = 0; i < hc[slot]; ++i) present[(size_t)(e - lo) * p->_nsrc + row[i]] = 1;
+    }
+  }
+  const int* ii = p->_ii->cpu_data();
+  const int* di = p->_di->cpu_data();
+  auto in_hist = [&](int e, int n) -> bool {
+    if (e < lo || e > x || e < 0) return false;
+    return present[(size_t)(e - lo) * p->_nsrc + n] != 0;
+  };
+  if (what == 0) {
+    auto& out = raw<int32_t>(v);
+    out.assign((size_t)R, 0);
+    for (int64_t r = 0; r < R; ++r) {
+      if (ii[r] < 0 || x < 0) continue;
+      const int d = di[r];
+      int q = 0;
+      for (int e = x - d + 1; e <= x; ++e)
+        if (in_hist(e, ii[r])) q |= 1 << ((d - 1) - (x - e));
+      out[(size_t)r] = q;
+    }
+  } else {
+    auto& out = raw<int8_t>(v);
+    out.assign((size_t)R, 0);
+    for (int64_t r = 0; r < R; ++r) {
+      if (ii[r] < 0 || x < 0) continue;
+      out[(size_t)r] = in_hist(x - di[r], ii[r]) ? 1 : 0;
+    }
+  }
+}
+
+/* ---- Pop ---- */
+
+void Pop::init_new(ProcParam proc_param, PopParam pop_param, Database& db, vector<Pop*>* list_pop,
+                   int* hcu_cnt, int* mcu_cnt) {
+  GlobalVar glv;
+  _id = (int)list_pop->size();
+  list_pop->push_back(this);
+  _dim_proj = 0;
+  _dim_hcu = pop_param.hcu_num;
+  _dim_mcu = pop_param.mcu_num;
+  if (_dim_mcu <= 0 || _dim_hcu <= 0) throw Error(-2, "pop_param: hcu_num and mcu_num must be > 0");
+  if (_dim_mcu > 1024) throw Error(-5, "mcu_num above 1024 is not supported");
+  _hcu_start = *hcu_cnt;
+  _mcu_start = *mcu_cnt;
+  *hcu_cnt += _dim_hcu;
+  *mcu_cnt += _dim_hcu * _dim_mcu;
+  float dt = 0;
+  int rank = 0;
+  GSBN_CHECK(glv.getf("dt", dt), "dt (ProcExtGen must come before ProcUpdLazy)");
+  GSBN_CHECK(glv.geti("rank", rank), "rank");
+  _rank = pop_param.rank;
+  _taumdt = dt / pop_param.taum;
+  _wtagain = pop_param.wtagain;
+  _maxfqdt = pop_param.maxfq * dt;
+  _igain = pop_param.igain;
+  _wgain = pop_param.wgain;
+  _lgbias = pop_param.lgbias;
+  _snoise = pop_param.snoise;
+  if (_rank != rank) throw Error(-5, "populations on other ranks are not supported (one process)");
+  const string sid = std::to_string(_id);
+  _dsup = db.create_sync_vector_f32("dsup_" + sid);
+  _act = db.create_sync_vector_f32("act_" + sid);
+  _ada = db.create_sync_vector_f32("ada_" + sid);
+  _epsc = db.create_sync_vector_f32("epsc_" + sid);
+  _bj = db.create_sync_vector_f32("bj_" + sid);
+  _spike = db.create_sync_vector_i8("spike_" + sid);
+  _rnd_uniform01 = db.create_sync_vector_f32(".rnd_uniform01_" + sid);
+  _rnd_normal = db.create_sync_vector_f32(".rnd_normal_" + sid);
+  _wmask = db.sync_vector_f32(".wmask");
+  _lginp = db.sync_vector_f32(".lginp");
+  _counter = db.create_sync_vector_i32(".counter_" + sid);
+  GSBN_CHECK(_dsup && _act && _ada && _epsc && _bj && _spike && _rnd_uniform01 && _rnd_normal &&
+                 _wmask && _lginp && _counter,
+             "Pop vectors");
+  const size_t N = (size_t)_dim_hcu * _dim_mcu;
+  _dsup->resize(N, (float)log(1.0 / _dim_mcu));
+  _act->resize(N, (float)(1.0 / _dim_mcu));
+  _ada->resize(N);
+  _spike->resize(N);
+  _rnd_uniform01->resize(N);
+  _rnd_normal->resize(N);
+  _counter->resize(N);
+  spk_list.alloc(N);
+  spk_cnt.alloc(_dim_hcu);
+  op_check(gsbn_i32_fill(spk_cnt.p, _dim_hcu, 0, active_stream()), "fill");
+  scratch.alloc(1);
+  /* external spikes for debugging (Pop.cpp:67-110): lines step,pop,idx,... */
+  Parser par(proc_param);
+  string filename;
+  if (par.args("external spike", filename)) {
+    std::ifstream f(filename);
+    if (!f) throw Error(-2, "Unable to open file to load external spikes");
+    std::map<int, vector<int>> ext;
+    string line;
+    while (std::getline(f, line)) {
+      std::stringstream ss(line);
+      vector<string> v;
+      while (ss.good()) {
+        string sub;
+        std::getline(ss, sub, ',');
+        v.push_back(sub);
+      }
+      if (v.size() > 2 && std::stoi(v[1]) == _id && std::stoi(v[0]) >= 0) {
+        vector<int> spk;
+        for (size_t i = 2; i < v.size(); i++) {
+          int u = std::stoi(v[i]);
+          if (u < 0 || (size_t)u >= N) throw Error(-2, "external spike unit out of range");
+          spk.push_back(u);
+        }
+        ext[std::stoi(v[0])] = spk;
+      }
+    }
+    has_ext = true;
+    ext_nsteps = ext.empty() ? 0 : ext.rbegin()->first + 1;
+    vector<int32_t> off(ext_nsteps + 1, 0), units;
+    for (int s = 0; s < ext_nsteps; ++s) {
+      auto it = ext.find(s);
+      if (it != ext.end()) units.insert(units.end(), it->second.begin(), it->second.end());
+      off[s + 1] = (int32_t)units.size();
+    }
+    ext_off.upload(off.data(), off.size());
+    if (units.empty()) units.push_back(0);
+    ext_units.upload(units.data(), units.size());
+  }
+}
+
+gsbn_pop_args Pop::args(int simstep, int exec, float prn, int lginp_idx, int wmask_idx, bool do_jxx,
+                        uint64_t seed, int wta_pop_max) {
+  gsbn_pop_args a;
+  memset(&a, 0, sizeof(a));
+  a.pop_id = _id;
+  a.H = _dim_hcu;
+  a.M = _dim_mcu;
+  a.simstep = simstep;
+  a.exec = exec;
+  a.seed = seed;
+  a.taumdt = _taumdt;
+  a.wtagain = _wtagain;
+  a.maxfqdt = _maxfqdt;
+  a.igain = _igain;
+  a.wgain = _wgain;
+  a.lgbias = _lgbias;
+  a.snoise = _snoise;
+  a.prn = prn;
+  a.wta_pop_max = wta_pop_max;
+  /* .lginp row lginp_idx and .wmask row wmask_idx (Pop.cpp:331-332) */
+  const size_t lg_off = _lginp->ld() <= 0 ? (size_t)lginp_idx : (size_t)lginp_idx * _lginp->ld();
+  const size_t wm_off = _wmask->ld() <= 0 ? (size_t)wmask_idx : (size_t)wmask_idx * _wmask->ld();
+  if (lginp_idx < 0 || lg_off + _mcu_start + (size_t)_dim_hcu * _dim_mcu > _lginp->size())
+    throw Error(-2, "lginp-idx out of range");
+  if (wmask_idx < 0 || wm_off + _hcu_start + (size_t)_dim_hcu > _wmask->size())
+    throw Error(-2, "wmask-idx " + std::to_string(wmask_idx) + " out of range");
+  a.lginp = _lginp->gpu_data(lginp_idx) + _mcu_start;
+  a.wmask = _wmask->gpu_data(wmask_idx) + _hcu_start;
+  a.dsup = _dsup->mutable_gpu_data();
+  a.act = _act->mutable_gpu_data();
+  a.spike = _spike->mutable_gpu_data();
+  a.counter = _counter->mutable_gpu_data();
+  a.rnd_u = _rnd_uniform01->mutable_gpu_data();
+  a.rnd_n = _rnd_normal->mutable_gpu_data();
+  a.epsc = _epsc->mutable_gpu_data();
+  a.bj = _bj->mutable_gpu_data();
+  a.scratch_dsup0 = scratch.p;
+  if (has_ext) {
+    a.ext_off = ext_off.p;
+    a.ext_units = ext_units.p;
+    a.ext_nsteps = ext_nsteps;
+  }
+  a.spk_list = spk_list.p;
+  a.spk_cnt = spk_cnt.p;
+  a.nproj_in = (int)_in.size();
+  a.do_jxx = do_jxx ? 1 : 0;
+  int max_slots = 0;
+  vector<gsbn_pop_proj> descs;
+  for (Proj* q : _in) {
+    descs.push_back(q->pop_desc());
+    max_slots = std::max(max_slots, q->_dim_conn);
+  }
+  a.max_slots = max_slots;
+  if (descs.size() != projs_host.size() ||
+      (descs.size() && memcmp(descs.data(), projs_host.data(), descs.size() * sizeof(gsbn_pop_proj)))) {
+    projs_host = descs;
+    projs_dev.upload(descs.data(), descs.size());
+  }
+  a.projs = projs_dev.p;
+  if (!work.p) {
+    work.alloc(2 * GSBN_MAX_PROJ_IN);
+    vector<int64_t> z(2 * GSBN_MAX_PROJ_IN, 0);
+    work.upload(z.data(), z.size());
+  }
+  a.work = work.p;
+  return a;
+}
+
+void Pop::update(int simstep, int exec, float prn, int lginp_idx, int wmask_idx, bool do_jxx,
+                 uint64_t seed, int wta_pop_max) {
+  gsbn_pop_args a = args(simstep, exec, prn, lginp_idx, wmask_idx, do_jxx, seed, wta_pop_max);
+  Timing::get().begin(0, active_stream());
+  op_check(gsbn_pop_update(&a, active_stream()), "gsbn_pop_update");
+  Timing::get().end(0, active_stream());
+  flush_pending = true;
+}
+
+/* apply pending support contributions before a host read of epsc */
+void Pop::flush() {
+  if (!flush_pending) return;
+  flush_pending = false;
+  int simstep = 0;
+  gsbn_pop_args a;
+  memset(&a, 0, sizeof(a));
+  a.H = _dim_hcu;
+  a.M = _dim_mcu;
+  a.epsc = _epsc->mutable_gpu_data();
+  a.nproj_in = (int)_in.size();
+  a.projs = projs_dev.p;
+  (void)simstep;
+  if (a.nproj_in && a.projs)
+    op_check(gsbn_pop_flush(&a, active_stream()), "gsbn_pop_flush");
+}
+
+/* ---- Proj ---- */
+
+void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
+                    vector<Proj*>* list_proj, vector<Pop*>* list_pop) {
+  (void)proc_param;
+  GlobalVar glv;
+  _id = (int)list_proj->size();
+  list_proj->push_back(this);
+  _ptr_src_pop = (*list_pop)[proj_param.src_pop];
+  _ptr_dest_pop = (*list_pop)[proj_param.dest_pop];
+  Pop* p = _ptr_dest_pop;
+  _dim_hcu = p->_dim_hcu;
+  _dim_mcu = p->_dim_mcu;
+  _nsrc = _ptr_src_pop->_dim_hcu * _ptr_src_pop->_dim_mcu;
+  _dim_conn = _nsrc;
+  if (_dim_conn > proj_param.slot_num) _dim_conn = (int)proj_param.slot_num;
+  if (_dim_conn <= 0) throw Error(-2, "proj_param: slot_num must be > 0");
+  _proj_in_pop = p->_dim_proj;
+  p->_dim_proj++;
+  if (p->_dim_proj > GSBN_MAX_PROJ_IN)
+    throw Error(-5, "more than 8 projections into one population are not supported");
+  p->_in.push_back(this);
+  _rank = p->_rank;
+  float dt = 0;
+  GSBN_CHECK(glv.getf("dt", dt), "dt");
+  if (!glv.geti("sim-end-step", sim_end)) sim_end = 0;
+  _tauzidt = dt / proj_param.tauzi;
+  _tauzjdt = dt / proj_param.tauzj;
+  _tauedt = dt / proj_param.taue;
+  _taupdt = dt / proj_param.taup;
+  _eps = dt / proj_param.taup;
+  _eps2 = (dt / proj_param.taup) * (dt / proj_param.taup);
+  _kfti = 1 / (proj_param.maxfq * proj_param.tauzi);
+  _kftj = 1 / (proj_param.maxfq * proj_param.tauzj);
+  _bgain = proj_param.bgain;
+  _wgain = proj_param.wgain;
+  _tauepscdt = proj_param.has_tauepsc() ? dt / proj_param.tauepsc : _tauzidt;
+
+  const size_t N = (size_t)_dim_hcu * _dim_mcu;
+  p->_epsc->resize((size_t)p->_dim_proj * N);
+  p->_bj->resize((size_t)p->_dim_proj * N);
+
+  const string sid = std::to_string(_id);
+  _ii = db.create_sync_vector_i32("ii_" + sid);
+  _qi = db.create_sync_vector_i32("qi_" + sid);
+  _di = db.create_sync_vector_i32("di_" + sid);
+  _ssi = db.create_sync_vector_i32(".ssi_" + sid);
+  _siq = db.create_sync_vector_i8(".siq_" + sid);
+  _pi = db.create_sync_vector_f32("pi_" + sid);
+  _ei = db.create_sync_vector_f32("ei_" + sid);
+  _zi = db.create_sync_vector_f32("zi_" + sid);
+  _ti = db.create_sync_vector_i32("ti_" + sid);
+  _ssj = db.create_sync_vector_i32(".ssj_" + sid);
+  _pj = db.create_sync_vector_f32("pj_" + sid);
+  _ej = db.create_sync_vector_f32("ej_" + sid);
+  _zj = db.create_sync_vector_f32("zj_" + sid);
+  _pij = db.create_sync_vector_f32("pij_" + sid);
+  _eij = db.create_sync_vector_f32("eij_" + sid);
+  _zi2 = db.create_sync_vector_f32("zi2_" + sid);
+  _zj2 = db.create_sync_vector_f32("zj2_" + sid);
+  _tij = db.create_sync_vector_i32("tij_" + sid);
+  _wij = db.create_sync_vector_f32("wij_" + sid);
+  _si = db.create_sync_vector_i8("si_" + sid);
+  GSBN_CHECK(_ii && _qi && _di && _ssi && _siq && _pi && _ei && _zi && _ti && _ssj && _pj && _ej &&
+                 _zj && _pij && _eij && _zi2 && _zj2 && _tij && _wij && _si,
+             "Proj vectors");
+
+  const int64_t R = (int64_t)_dim_hcu * _dim_conn;
+  const int64_t S = R * _dim_mcu;
+  _ii->resize(R, -1);
+  _di->resize(R);
+  _pj->resize(N, (float)(1.0 / _dim_mcu));
+  _ej->resize(N);
+  _zj->resize(N);
+  _si->resize(_nsrc);
+
+  /* device state, initial values of Proj.cpp:88-105 */
+  syn.alloc((size_t)S);
+  rowst.alloc((size_t)R);
+  gsbn_syn_rec s0;
+  memset(&s0, 0, sizeof(s0));
+  s0.pij = (float)(1.0 / _dim_conn / _dim_mcu);
+  op_check(gsbn_syn_fill(syn.p, S, s0, active_stream()), "gsbn_syn_fill");
+  gsbn_row_rec r0;
+  memset(&r0, 0, sizeof(r0));
+  r0.pi = (float)(1.0 / _dim_conn);
+  op_check(gsbn_row_fill(rowst.p, R, r0, active_stream()), "gsbn_row_fill");
+  contrib.alloc((size_t)S);
+  rows_list.alloc((size_t)R);
+  rows_cnt.alloc(_dim_hcu);
+  pend_cnt.alloc(_dim_hcu);
+  act_stamp.alloc((size_t)R);
+  hist_list.alloc((size_t)GSBN_HIST * std::max(_nsrc, 1));
+  hist_cnt.alloc(GSBN_HIST);
+  op_check(gsbn_i32_fill(rows_cnt.p, _dim_hcu, 0, active_stream()), "fill");
+  op_check(gsbn_i32_fill(pend_cnt.p, _dim_hcu, 0, active_stream()), "fill");
+  op_check(gsbn_i32_fill(act_stamp.p, R, -1, active_stream()), "fill");
+  op_check(gsbn_i32_fill(hist_cnt.p, GSBN_HIST, 0, active_stream()), "fill");
+
+  /* views over the device state */
+  struct VB {
+    VecBase* v;
+    VecView* view;
+    size_t n;
+  };
+  const VB vb[] = {
+      {_pij, new SynField(this, 0), (size_t)S}, {_eij, new SynField(this, 1), (size_t)S},
+      {_zi2, new SynField(this, 2), (size_t)S}, {_zj2, new SynField(this, 3), (size_t)S},
+      {_wij, new SynField(this, 4), (size_t)S}, {_tij, new SynField(this, 5), (size_t)S},
+      {_pi, new RowField(this, 0), (size_t)R},  {_ei, new RowField(this, 1), (size_t)R},
+      {_zi, new RowField(this, 2), (size_t)R},  {_ti, new RowField(this, 3), (size_t)R},
+      {_qi, new QueueView(this, 0), (size_t)R}, {_siq, new QueueView(this, 1), (size_t)R},
+      {_ssi, new QueueView(this, 2), 0},        {_ssj, new QueueView(this, 3), 0},
+  };
+  for (auto& e : vb) {
+    view_objs.emplace_back(e.view);
+    e.v->view = e.view;
+    if (e.v->type == DT_F32) static_cast<SyncVector<float>*>(e.v)->set_logical_size(e.n);
+    if (e.v->type == DT_I32) static_cast<SyncVector<int32_t>*>(e.v)->set_logical_size(e.n);
+    if (e.v->type == DT_I8) static_cast<SyncVector<int8_t>*>(e.v)->set_logical_size(e.n);
+    e.v->status = VecBase::DEVICE_NEWER;
+    views.push_back(e.v);
+  }
+  Pop* dp = _ptr_dest_pop;
+  _ptr_dest_pop->_epsc->before_host_read = [dp]() { dp->flush(); };
+}
+
+gsbn_pop_proj Proj::pop_desc() {
+  gsbn_pop_proj d;
+  memset(&d, 0, sizeof(d));
+  d.contrib = contrib.p;
+  d.pend_cnt = pend_cnt.p;
+  d.C = _dim_conn;
+  d.pj = _pj->mutable_gpu_data();
+  d.ej = _ej->mutable_gpu_data();
+  d.zj = _zj->mutable_gpu_data();
+  d.taupdt = _taupdt;
+  d.ke = _tauedt;
+  d.kzj = _tauzjdt;
+  d.kepsc = _tauepscdt;
+  d.kftj = _kftj;
+  d.bgain = _bgain;
+  d.eps = _eps;
+  d.hist_list = hist_list.p;
+  d.hist_cnt = hist_cnt.p;
+  d.Nsrc = _nsrc;
+  d.dmin = dmin;
+  d.dmax = dmax;
+  d.inv_off = inv_off.p;
+  d.inv_slot = inv_slot.p;
+  d.di = _di->gpu_data();
+  d.rows_list = rows_list.p;
+  d.rows_cnt = rows_cnt.p;
+  d.act_stamp = act_stamp.p;
+  return d;
+}
+
+/* inverse connectivity per (destination HCU, source unit), slots ascending */
+void Proj::rebuild_fanout() {
+  const int H = _dim_hcu, C = _dim_conn;
+  const int* ii = _ii->cpu_data();
+  const int* di = _di->cpu_data();
+  vector<int32_t> off((size_t)H * (_nsrc + 1), 0);
+  vector<int32_t> slots;
+  slots.reserve((size_t)H * C);
+  dmin = 32;
+  dmax = 0;
+  for (int h = 0; h < H; ++h) {
+    int32_t* o = off.data() + (size_t)h * (_nsrc + 1);
+    vector<int32_t> cnt(_nsrc, 0);
+    for (int k = 0; k < C; ++k) {
+      const int n = ii[(size_t)h * C + k];
+      if (n < 0) continue;
+      if (n >= _nsrc) throw Error(-2, "ii_" + std::to_string(_id) + ": source unit out of range");
+      const int d = di[(size_t)h * C + k];
+      if (d < 1 || d > 31) throw Error(-2, "di_" + std::to_string(_id) + ": delay must be in [1, 31]");
+      cnt[n]++;
+      dmin = std::min(dmin, d);
+      dmax = std::max(dmax, d);
+    }
+    const int base = (int)slots.size();
+    o[0] = base;
+    for (int n = 0; n < _nsrc; ++n) o[n + 1] = o[n] + cnt[n];
+    slots.resize(base + (size_t)(o[_nsrc] - base));
+    vector<int32_t> fill(o, o + _nsrc);
+    for (int k = 0; k < C; ++k) {
+      const int n = ii[(size_t)h * C + k];
+      if (n < 0) continue;
+      slots[fill[n]++] = k;
+    }
+  }
+  if (dmax == 0) {
+    dmin = 1;
+    dmax = 0; /* no connections: empty delay range */
+  }
+  inv_off.upload(off.data(), off.size());
+  if (slots.empty()) slots.push_back(0);
+  inv_slot.upload(slots.data(), slots.size());
+}
+
+const CoefTable& Proj::table(float kp, int64_t need) {
+  uint32_t key;
+  memcpy(&key, &kp, 4);
+  auto& slot = tables[key];
+  if (!slot) slot.reset(new CoefTable());
+  CoefTable& t = *slot;
+  if (t.saturated || t.len >= need) return t;
+  gsbn_rates_t r = {(double)_tauzidt, (double)_tauzjdt, (double)_tauedt, (double)kp};
+  const int64_t cap = (int64_t)1 << 26;
+  int64_t want = std::max<int64_t>(need, std::max<int64_t>(2 * t.len, (int64_t)sim_end + 2));
+  const int64_t sat = gsbn_coeff_saturation(&r, cap);
+  bool saturated = false;
+  if (sat >= 0 && sat + 1 <= want) {
+    want = sat + 1;
+    saturated = true;
+  }
+  if (want > cap) throw Error(-5, "coefficient table beyond 2^26 gaps");
+  vector<float> host((size_t)want * GSBN_COEF_STRIDE);
+  for (int64_t d = 0; d < want; ++d) gsbn_coeff_entry(&r, d, host.data() + d * GSBN_COEF_STRIDE);
+  t.dev.upload(host.data(), host.size());
+  t.len = want;
+  t.saturated = saturated;
+  return t;
+}
+
+void Proj::prepare(int simstep) {
+  for (VecBase* v : views) {
+    if (v->status == VecBase::HOST_NEWER && v->view) {
+      v->view->push(v);
+      v->status = VecBase::SYNCED;
+    }
+  }
+  if (_ii->host_version != ii_version || _di->host_version != di_version) {
+    rebuild_fanout();
+    ii_version = _ii->host_version;
+    di_version = _di->host_version;
+  }
+  (void)_ii->gpu_data();
+  (void)_di->gpu_data();
+  (void)simstep;
+}
+
+void Proj::update_all(int simstep, float old_prn) {
+  const float kp = _taupdt * old_prn;
+  const int target = simstep - 1;
+  const CoefTable& t = table(kp, (int64_t)std::max(target, 0) + 2);
+  gsbn_all_args a;
+  memset(&a, 0, sizeof(a));
+  a.H = _dim_hcu;
+  a.M = _dim_mcu;
+  a.C = _dim_conn;
+  a.target = target;
+  a.kp = kp;
+  a.wgain = _wgain;
+  a.eps = _eps;
+  a.eps2 = _eps2;
+  a.table = t.dev.p;
+  a.table_len = (int32_t)t.len;
+  a.syn = syn.p;
+  a.rowst = rowst.p;
+  a.pj = _pj->gpu_data();
+  Timing::get().begin(2, active_stream());
+  op_check(gsbn_proj_all(&a, active_stream()), "gsbn_proj_all");
+  Timing::get().end(2, active_stream());
+}
+
+void Proj::update_jxx(float prn) {
+  Pop* dp = _ptr_dest_pop;
+  gsbn_jxx_args a;
+  memset(&a, 0, sizeof(a));
+  const int64_t N = (int64_t)_dim_hcu * _dim_mcu;
+  a.N = (int32_t)N;
+  a.sj = dp->_spike->gpu_data();
+  a.pj = _pj->mutable_gpu_data();
+  a.ej = _ej->mutable_gpu_data();
+  a.zj = _zj->mutable_gpu_data();
+  a.bj = dp->_bj->mutable_gpu_data() + _proj_in_pop * N;
+  a.epsc = dp->_epsc->mutable_gpu_data() + _proj_in_pop * N;
+  a.kp = _taupdt * prn;
+  a.ke = _tauedt;
+  a.kzj = _tauzjdt;
+  a.kepsc = _tauepscdt;
+  a.kftj = _kftj;
+  a.bgain = _bgain;
+  a.eps = _eps;
+  Timing::get().begin(3, active_stream());
+  op_check(gsbn_proj_jxx(&a, active_stream()), "gsbn_proj_jxx");
+  Timing::get().end(3, active_stream());
+}
+
+void Proj::update_rowcol(int simstep, int exec, float prn) {
+  Pop* dp = _ptr_dest_pop;
+  const float kp = _taupdt * prn;
+  const CoefTable& t = table(kp, (int64_t)simstep + 2);
+  gsbn_rowcol_args a;
+  memset(&a, 0, sizeof(a));
+  a.H = _dim_hcu;
+  a.M = _dim_mcu;
+  a.C = _dim_conn;
+  a.simstep = simstep;
+  a.exec = exec;
+  a.kp = kp;
+  a.kfti = _kfti;
+  a.kftj = _kftj;
+  a.wgain = _wgain;
+  a.eps = _eps;
+  a.eps2 = _eps2;
+  a.table = t.dev.p;
+  a.table_len = (int32_t)t.len;
+  a.syn = syn.p;
+  a.rowst = rowst.p;
+  a.pj = _pj->gpu_data();
+  a.sj = dp->_spike->gpu_data();
+  a.ii = _ii->gpu_data();
+  a.rows_list = rows_list.p;
+  a.rows_cnt = rows_cnt.p;
+  a.spk_list = dp->spk_list.p;
+  a.spk_cnt = dp->spk_cnt.p;
+  a.act_stamp = act_stamp.p;
+  a.contrib = contrib.p;
+  Timing::get().begin(1, active_stream());
+  op_check(gsbn_proj_rowcol(&a, active_stream()), "gsbn_proj_rowcol");
+  Timing::get().end(1, active_stream());
+}
+
+void Proj::update_que(int exec) {
+  gsbn_que_args a;
+  a.Nsrc = _nsrc;
+  a.exec = exec;
+  a.si = _si->gpu_data();
+  a.hist_list = hist_list.p;
+  a.hist_cnt = hist_cnt.p;
+  Timing::get().begin(4, active_stream());
+  op_check(gsbn_proj_que(&a, active_stream()), "gsbn_proj_que");
+  Timing::get().end(4, active_stream());
+  last_exec = exec;
+}
+
+void Proj::mark_views_device() {
+  for (VecBase* v : views) v->status = VecBase::DEVICE_NEWER;
+}
+
+/* ---- the procedure ---- */
+
+class ProcUpdLazy : public ProcedureBase {
+  REGISTER(ProcUpdLazy)
+ public:
+  ~ProcUpdLazy() override;
+  void init_new(SolverParam solver_param, Database& db) override;
+  void init_copy(SolverParam solver_param, Database& db) override;
+  void update_cpu() override;
+  void update_gpu() override;
+  int64_t work(int i);
+  int64_t sweep_synapses = 0, sweeps = 0, execs = 0;
+
+ private:
+  vector<Proj*> _list_proj;
+  vector<Pop*> _list_pop;
+  GlobalVar _glv;
+  uint64_t _seed = 0;
+  int _wta_pop_max = 0;
+  int _exec = 0;
+};
+REGISTERIMPL(ProcUpdLazy);
+
+ProcUpdLazy::~ProcUpdLazy() {
+  for (auto* p : _list_proj) delete p;
+  for (auto* p : _list_pop) delete p;
+}
+
+void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
+  NetParam net_param = solver_param.net_param;
+  ProcParam proc_param = get_proc_param(solver_param);
+  Parser par(proc_param);
+  int lo = 0, hi = 0;
+  par.argi("seed-lo", lo);
+  par.argi("seed-hi", hi);
+  _seed = (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 31);
+  par.argi("wta-pop-max", _wta_pop_max);
+  int hcu_cnt = 0, mcu_cnt = 0;
+  for (auto& pp : net_param.pop_param) {
+    for (uint32_t j = 0; j < pp.pop_num; j++) {
+      Pop* p = new Pop();
+      try {
+        p->init_new(proc_param, pp, db, &_list_pop, &hcu_cnt, &mcu_cnt);
+      } catch (...) {
+        if (_list_pop.empty() || _list_pop.back() != p) delete p;
+        throw;
+      }
+    }
+  }
+  const int total_pop_num = (int)_list_pop.size();
+  for (auto& jp : net_param.proj_param) {
+    if ((int)jp.src_pop < total_pop_num && (int)jp.dest_pop < total_pop_num) {
+      Proj* proj = new Proj();
+      try {
+        proj->init_new(proc_param, jp, db, &_list_proj, &_list_pop);
+      } catch (...) {
+        if (_list_proj.empty() || _list_proj.back() != proj) delete proj;
+        throw;
+      }
+    }
+  }
+  hip_check(hipStreamSynchronize(active_stream()), "sync");
+}
+
+void ProcUpdLazy::init_copy(SolverParam, Database&) {
+  throw Error(-5, "ProcUpdLazy::init_copy: snapshot resume is not supported by this build");
+}
+
+void ProcUpdLazy::update_cpu() {
+  throw Error(-5, "ProcUpdLazy::update_cpu: this build runs the lazy update on the GPU only");
+}
+
+void ProcUpdLazy::update_gpu() {
+  int cycle_flag = 0;
+  GSBN_CHECK(_glv.geti("cycle-flag", cycle_flag), "cycle-flag");
+  if (cycle_flag != 1) return;
+  int simstep = 0, lginp_idx = 0, wmask_idx = 0;
+  float prn = 0, old_prn = 0;
+  GSBN_CHECK(_glv.geti("simstep", simstep), "simstep");
+  GSBN_CHECK(_glv.getf("prn", prn), "prn");
+  GSBN_CHECK(_glv.getf("old-prn", old_prn), "old-prn");
+  GSBN_CHECK(_glv.geti("lginp-idx", lginp_idx), "lginp-idx");
+  GSBN_CHECK(_glv.geti("wmask-idx", wmask_idx), "wmask-idx");
+  for (auto* q : _list_proj) q->prepare(simstep);
+  const bool sweep = old_prn != prn;
+  for (auto* p : _list_pop)
+    p->update(simstep, _exec, prn, lginp_idx, wmask_idx, !sweep, _seed, _wta_pop_max);
+  if (sweep) {
+    for (auto* q : _list_proj) {
+      q->update_all(simstep, old_prn);
+      sweep_synapses += (int64_t)q->_dim_hcu * q->_dim_conn * q->_dim_mcu;
+    }
+    sweeps++;
+    for (auto* q : _list_proj) q->update_jxx(prn);
+  }
+  for (auto* q : _list_proj) q->update_rowcol(simstep, _exec, prn);
+  for (auto* q : _list_proj) q->update_que(_exec);
+  for (auto* q : _list_proj) q->mark_views_device();
+  _exec++;
+  execs++;
+}
+
+/* counters: 0 row synapse updates, 1 column synapse updates */
+int64_t ProcUpdLazy::work(int i) {
+  int64_t total = 0;
+  for (auto* p : _list_pop) {
+    if (!p->work.p) continue;
+    vector<int64_t> w(2 * GSBN_MAX_PROJ_IN);
+    p->work.download(w.data(), w.size());
+    for (size_t m = 0; m < p->_in.size(); ++m) total += w[2 * m + i];
+  }
+  return total;
+}
+
+}  // namespace proc_upd_lazy
+
+/* accessors used by the C ABI */
+bool updlazy_work(Upd& upd, int64_t* out5) {
+  for (auto* p : upd.procedures()) {
+    auto* u = dynamic_cast<proc_upd_lazy::ProcUpdLazy*>(p);
+    if (u) {
+      out5[0] = u->work(0);
+      out5[1] = u->work(1);
+      out5[2] = u->sweep_synapses;
+      out5[3] = u->execs;
+      out5[4] = u->sweeps;
+      return true;
+    }
+  }
+  return false;
+}
+
+}  // namespace gsbn

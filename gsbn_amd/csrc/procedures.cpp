@@ -1,0 +1,492 @@
+/*
+ * procedures.cpp -- the procedures around ProcUpdLazy on its data path, with
+ * the reference's names, Database/GlobalVar contracts and error cases:
+ *
+ *   ProcExtGen         simulation clock, mode schedule, .lginp/.wmask/.lgidx
+ *                      (ProcExtGen.cpp:8-153)
+ *   ProcInitConn       ii/di from a connection map (ProcInitConn.cpp:8-103)
+ *   ProcExchangeSpike  si_<proj> <- spike_<src> once per executed step; one
+ *                      process, so the exchange is a device copy
+ *                      (ProcExchangeSpike.cpp:94-135)
+ *   ProcCheck          recall scoring (ProcCheck.cpp:8-237,
+ *                      ProcCheck/Pop.cpp:35-60); counting and the per-HCU
+ *                      argmax run on the device
+ */
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "../../include/gsbn_amd.h"
+#include "detmath.h"
+#include "framework.h"
+
+namespace gsbn {
+
+/* ------------------------------------------------------------------ */
+class ProcExtGen : public ProcedureBase {
+  REGISTER(ProcExtGen)
+ public:
+  struct mode_t {
+    int begin_step, end_step, lgidx_id, lgexp_id, wmask_id;
+    float prn;
+    int plasticity;
+  };
+  void init_new(SolverParam solver_param, Database& db) override;
+  void init_copy(SolverParam solver_param, Database& db) override { init_new(solver_param, db); }
+  void update_cpu() override;
+  void update_gpu() override { update_cpu(); }
+
+ private:
+  GlobalVar _glv;
+  vector<mode_t> _list_mode;
+  SyncVector<int32_t>* _lgidx = nullptr;
+  SyncVector<float>* _wmask = nullptr;
+  SyncVector<float>* _lginp = nullptr;
+  vector<int> _mcu_in_hcu;
+  float _eps = 0;
+  int _cursor = 0;
+  int _old_lgidx_id = -1;
+};
+REGISTERIMPL(ProcExtGen);
+
+void ProcExtGen::init_new(SolverParam solver_param, Database& db) {
+  GenParam gen_param = solver_param.gen_param;
+  _eps = gen_param.eps;
+  float dt = gen_param.dt;
+  _glv.putf("dt", dt);
+  int max_step = -1;
+  int last_end = 0;
+  for (auto& mp : gen_param.mode_param) {
+    int begin_step = int(mp.begin_time / dt);
+    int end_step = int(mp.end_time / dt);
+    if (begin_step < max_step)
+      throw Error(-2, "Order of modes is wrong or there is overlapping time range, abort!");
+    if (end_step < begin_step) throw Error(-2, "Time range is wrong, abort!");
+    int lg = mp.begin_lgidx_id, lx = mp.begin_lgexp_id, wm = mp.begin_wmask_id;
+    while (begin_step < end_step) {
+      mode_t m;
+      m.begin_step = begin_step;
+      m.end_step = begin_step + (int)mp.time_step;
+      if (m.end_step >= end_step) m.end_step = end_step;
+      m.lgidx_id = lg;
+      m.lgexp_id = lx;
+      m.wmask_id = wm;
+      lg += mp.lgidx_step;
+      lx += mp.lgexp_step;
+      wm += mp.wmask_step;
+      m.prn = mp.prn;
+      m.plasticity = mp.plasticity;
+      _list_mode.push_back(m);
+      begin_step = m.end_step;
+    }
+    if (end_step > last_end) last_end = end_step;
+  }
+  /* horizon of the schedule, used to size the coefficient tables */
+  _glv.puti("sim-end-step", last_end);
+
+  _lgidx = db.create_sync_vector_i32(".lgidx");
+  _wmask = db.create_sync_vector_f32(".wmask");
+  _lginp = db.create_sync_vector_f32(".lginp");
+  GSBN_CHECK(_lgidx && _wmask && _lginp, "ProcExtGen vectors already exist");
+
+  StimRawData stim = read_stim_raw_data(gen_param.stim_file);
+  if (stim.data.size() != (size_t)stim.data_rows * stim.data_cols)
+    throw Error(-2, "Bad stimuli!!! (data size)");
+  if (stim.mask.size() != (size_t)stim.mask_rows * stim.mask_cols)
+    throw Error(-2, "Bad stimuli!!! (mask size)");
+  _lgidx->set_ld(stim.data_cols);
+  _wmask->set_ld(stim.mask_cols);
+  auto* vd = _lgidx->mutable_cpu_vector();
+  vd->clear();
+  for (uint32_t v : stim.data) vd->push_back((int32_t)v);
+  _lgidx->resize(vd->size());
+  auto* vm = _wmask->mutable_cpu_vector();
+  vm->clear();
+  for (float v : stim.mask) vm->push_back(v);
+  _wmask->resize(vm->size());
+
+  int mcu_num = 0;
+  for (auto& pp : solver_param.net_param.pop_param) {
+    mcu_num += pp.hcu_num * pp.mcu_num * pp.pop_num;
+    for (uint32_t k = 0; k < pp.pop_num; ++k)
+      for (uint32_t j = 0; j < pp.hcu_num; ++j) _mcu_in_hcu.push_back(pp.mcu_num);
+  }
+  _lginp->resize(mcu_num);
+  _cursor = 0;
+  _old_lgidx_id = -1;
+}
+
+void ProcExtGen::update_cpu() {
+  int current_step = 0;
+  GSBN_CHECK(_glv.geti("simstep", current_step), "simstep");
+  current_step++;
+  _glv.puti("simstep", current_step);
+  int r = (int)_list_mode.size();
+  for (; _cursor < r; _cursor++) {
+    mode_t m = _list_mode[_cursor];
+    if (current_step > m.begin_step && current_step <= m.end_step) {
+      float old_prn = 0;
+      GSBN_CHECK(_glv.getf("prn", old_prn), "prn");
+      _glv.putf("old-prn", old_prn);
+      _glv.putf("prn", m.prn);
+      _glv.puti("wmask-idx", m.wmask_id);
+      _glv.putb("plasticity", (bool)m.plasticity);
+      _glv.puti("lginp-idx", 0);
+      _glv.puti("cycle-flag", 1);
+      if (_old_lgidx_id != m.lgidx_id) {
+        _old_lgidx_id = m.lgidx_id;
+        if ((size_t)(m.lgidx_id + 1) * _lgidx->ld() > _lgidx->size())
+          throw Error(-2, "stimulus row " + std::to_string(m.lgidx_id) + " out of range");
+        const int* lgidx = _lgidx->cpu_data(m.lgidx_id);
+        float* lginp = _lginp->mutable_cpu_data();
+        const float on = gsbn_logf(1 + _eps), off = gsbn_logf(0 + _eps);
+        for (size_t i = 0; i < _mcu_in_hcu.size(); i++) {
+          for (int j = 0; j < _mcu_in_hcu[i]; j++) {
+            *lginp = (lgidx[i] == j) ? on : off;
+            lginp++;
+          }
+        }
+      }
+      return;
+    } else if (current_step <= m.begin_step) {
+      _glv.puti("cycle-flag", 0);
+      return;
+    }
+  }
+  _glv.puti("cycle-flag", -1);
+}
+
+/* ------------------------------------------------------------------ */
+class ProcInitConn : public ProcedureBase {
+  REGISTER(ProcInitConn)
+ public:
+  void init_new(SolverParam solver_param, Database& db) override;
+  void init_copy(SolverParam solver_param, Database& db) override { init_new(solver_param, db); }
+  void update_cpu() override {}
+  void update_gpu() override {}
+};
+REGISTERIMPL(ProcInitConn);
+
+void ProcInitConn::init_new(SolverParam solver_param, Database& db) {
+  struct Prj {
+    int dim_conn = 0, dim_hcu = 0, nsrc = 0;
+    SyncVector<int32_t>* ii = nullptr;
+    SyncVector<int32_t>* di = nullptr;
+  };
+  vector<std::pair<int, int>> pops;  // hcu, mcu
+  for (auto& pp : solver_param.net_param.pop_param)
+    for (uint32_t k = 0; k < pp.pop_num; ++k) pops.push_back({(int)pp.hcu_num, (int)pp.mcu_num});
+  vector<Prj> prjs;
+  for (auto& jp : solver_param.net_param.proj_param) {
+    if (jp.src_pop >= pops.size() || jp.dest_pop >= pops.size()) continue;
+    Prj p;
+    p.dim_hcu = pops[jp.dest_pop].first;
+    p.nsrc = pops[jp.src_pop].first * pops[jp.src_pop].second;
+    p.dim_conn = p.nsrc;
+    if (p.dim_conn > jp.slot_num) p.dim_conn = (int)jp.slot_num;
+    int id = (int)prjs.size();
+    p.ii = db.sync_vector_i32("ii_" + std::to_string(id));
+    p.di = db.sync_vector_i32("di_" + std::to_string(id));
+    prjs.push_back(p);
+  }
+  Parser par(get_proc_param(solver_param));
+  string conn_map_file;
+  if (!par.args("conn-map-file", conn_map_file))
+    throw Error(-2, "Connection map is not specified! Abort");
+  std::ifstream f(conn_map_file);
+  if (!f) throw Error(-2, "cannot open connection map " + conn_map_file);
+  string line;
+  while (std::getline(f, line)) {
+    std::stringstream ss(line);
+    vector<string> v;
+    while (ss.good()) {
+      string sub;
+      std::getline(ss, sub, ',');
+      v.push_back(sub);
+    }
+    if (v.size() <= 2) continue;
+    int prj_id = std::stoi(v[0]);
+    if (prj_id < 0 || prj_id >= (int)prjs.size()) throw Error(-2, "conn-map: bad projection id");
+    Prj& p = prjs[prj_id];
+    if (!p.ii || !p.di) throw Error(-4, "conn-map: ii/di vectors missing (ProcUpdLazy must come first)");
+    int hcu_id = std::stoi(v[1]);
+    if (hcu_id < 0 || hcu_id >= p.dim_hcu) throw Error(-2, "conn-map: bad destination hcu");
+    if ((v.size() - 2) % 2) throw Error(-2, "conn-map: unpaired source/delay");
+    int* pii = p.ii->mutable_cpu_data() + (int64_t)hcu_id * p.dim_conn;
+    int* pdi = p.di->mutable_cpu_data() + (int64_t)hcu_id * p.dim_conn;
+    for (size_t i = 2; i + 1 < v.size(); i += 2) {
+      int ii = std::stoi(v[i]);
+      int di = std::stoi(v[i + 1]);
+      if (ii < 0 || ii >= p.nsrc) throw Error(-2, "conn-map: source unit out of range");
+      if (di < 1 || di > 31) throw Error(-2, "conn-map: delay must be in [1, 31]");
+      /* ProcInitConn::add_row: first free slot of the hypercolumn */
+      for (int k = 0; k < p.dim_conn; k++) {
+        if (pii[k] < 0) {
+          pii[k] = ii;
+          pdi[k] = di;
+          break;
+        }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+class ProcExchangeSpike : public ProcedureBase {
+  REGISTER(ProcExchangeSpike)
+ public:
+  void init_new(SolverParam solver_param, Database& db) override;
+  void init_copy(SolverParam solver_param, Database& db) override { init_new(solver_param, db); }
+  void update_cpu() override { update_gpu(); }
+  void update_gpu() override;
+
+ private:
+  GlobalVar _glv;
+  struct Link {
+    SyncVector<int8_t>* sj;
+    SyncVector<int8_t>* si;
+  };
+  vector<Link> _links;
+};
+REGISTERIMPL(ProcExchangeSpike);
+
+void ProcExchangeSpike::init_new(SolverParam solver_param, Database& db) {
+  int rank = 0, num_rank = 1;
+  GSBN_CHECK(_glv.geti("rank", rank), "rank");
+  GSBN_CHECK(_glv.geti("num-rank", num_rank), "num-rank");
+  if (num_rank != 1) throw Error(-5, "ProcExchangeSpike: one process only in this build");
+  int npop = 0;
+  for (auto& pp : solver_param.net_param.pop_param) npop += pp.pop_num;
+  int id = 0;
+  for (auto& jp : solver_param.net_param.proj_param) {
+    if ((int)jp.src_pop >= npop || (int)jp.dest_pop >= npop) continue;
+    Link l;
+    l.sj = db.sync_vector_i8("spike_" + std::to_string(jp.src_pop));
+    l.si = db.sync_vector_i8("si_" + std::to_string(id));
+    GSBN_CHECK(l.sj && l.si, "ProcExchangeSpike: spike/si vectors missing");
+    GSBN_CHECK(l.si->size() == l.sj->size(), "ProcExchangeSpike: si size");
+    _links.push_back(l);
+    ++id;
+  }
+}
+
+void ProcExchangeSpike::update_gpu() {
+  int cycle_flag = 0;
+  GSBN_CHECK(_glv.geti("cycle-flag", cycle_flag), "cycle-flag");
+  if (cycle_flag != 1) return;
+  for (auto& l : _links) {
+    const int8_t* src = l.sj->gpu_data();
+    int8_t* dst = l.si->mutable_gpu_data();
+    if (l.sj->size())
+      hip_check(hipMemcpyAsync(dst, src, l.sj->size(), hipMemcpyDeviceToDevice, active_stream()),
+                "exchange copy");
+  }
+}
+
+/* ------------------------------------------------------------------ */
+class ProcCheck : public ProcedureBase {
+  REGISTER(ProcCheck)
+ public:
+  ~ProcCheck() override;
+  void init_new(SolverParam solver_param, Database& db) override;
+  void init_copy(SolverParam solver_param, Database& db) override { init_new(solver_param, db); }
+  void update_cpu() override { update_gpu(); }
+  void update_gpu() override;
+  void finish() override { report(false); }
+  void results(int* correct, int* total, vector<int>* flags);
+
+ private:
+  struct mode_t {
+    int begin_step, end_step, lgexp_id;
+  };
+  struct PopInfo {
+    int H, M;
+    SyncVector<int8_t>* spike;
+    int32_t* count;
+  };
+  void report(bool final_line);
+  GlobalVar _glv;
+  vector<mode_t> _list_mode;
+  SyncVector<int32_t>* _lgidx = nullptr;
+  vector<PopInfo> _pops;
+  int _total_hcu = 0;
+  int _threshold = 0;
+  int _cursor = 0;
+  bool _updated_flag = false;
+  string _logfile;
+  int32_t* _res_idx = nullptr; /* [windows][total_hcu] */
+  int32_t* _res_cnt = nullptr;
+  vector<int> _window_mode;    /* mode index of each scored window */
+  int _reported = 0;
+  int _correct = 0, _total = 0;
+  vector<int> _flags;
+  bool _final_done = false;
+};
+REGISTERIMPL(ProcCheck);
+
+ProcCheck::~ProcCheck() {
+  for (auto& p : _pops)
+    if (p.count) (void)hipFree(p.count);
+  if (_res_idx) (void)hipFree(_res_idx);
+  if (_res_cnt) (void)hipFree(_res_cnt);
+}
+
+void ProcCheck::init_new(SolverParam solver_param, Database& db) {
+  string log_dir;
+  GSBN_CHECK(_glv.gets("log-dir", log_dir) && !log_dir.empty(), "log-dir");
+  string dir = log_dir + proc_name();
+  string cmd = "mkdir -p '" + dir + "'";
+  if (system(cmd.c_str()) != 0) throw Error(-2, "Cannot create directory for state records!");
+  _logfile = dir + "/check_result.txt";
+  GenParam gen_param = solver_param.gen_param;
+  float dt = gen_param.dt;
+  int max_step = -1;
+  for (auto& mp : gen_param.mode_param) {
+    int begin_step = int(mp.begin_time / dt);
+    int end_step = int(mp.end_time / dt);
+    if (begin_step < max_step) throw Error(-2, "Order of modes is wrong");
+    if (end_step < begin_step) throw Error(-2, "Time range is wrong, abort!");
+    if (mp.prn != 0.0f) continue;
+    int lx = mp.begin_lgexp_id;
+    while (begin_step < end_step) {
+      mode_t m;
+      m.begin_step = begin_step;
+      m.end_step = begin_step + (int)mp.time_step;
+      m.lgexp_id = lx;
+      lx += mp.lgexp_step;
+      _list_mode.push_back(m);
+      begin_step += mp.time_step;
+    }
+  }
+  _lgidx = db.sync_vector_i32(".lgidx");
+  GSBN_CHECK(_lgidx, ".lgidx missing (ProcExtGen must come first)");
+  int pid = 0;
+  for (auto& pp : solver_param.net_param.pop_param) {
+    for (uint32_t k = 0; k < pp.pop_num; ++k) {
+      PopInfo pi;
+      pi.H = pp.hcu_num;
+      pi.M = pp.mcu_num;
+      pi.spike = db.sync_vector_i8("spike_" + std::to_string(pid));
+      GSBN_CHECK(pi.spike, "spike vector missing");
+      hip_check(hipMalloc(&pi.count, sizeof(int32_t) * pi.H * pi.M), "hipMalloc");
+      op_check(gsbn_i32_fill(pi.count, (int64_t)pi.H * pi.M, 0, active_stream()), "gsbn_i32_fill");
+      _total_hcu += pi.H;
+      _pops.push_back(pi);
+      ++pid;
+    }
+  }
+  Parser par(get_proc_param(solver_param));
+  if (!par.argi("threashold", _threshold)) _threshold = 0;
+  size_t nwin = _list_mode.size() ? _list_mode.size() : 1;
+  hip_check(hipMalloc(&_res_idx, sizeof(int32_t) * nwin * _total_hcu), "hipMalloc");
+  hip_check(hipMalloc(&_res_cnt, sizeof(int32_t) * nwin * _total_hcu), "hipMalloc");
+  std::fstream out(_logfile, std::ios::out | std::ofstream::trunc);
+  out << "result | reference pattern | recorded pattern | recorded pattern spike count" << std::endl;
+  _cursor = 0;
+  _updated_flag = false;
+}
+
+void ProcCheck::update_gpu() {
+  int cycle_flag = 0;
+  GSBN_CHECK(_glv.geti("cycle-flag", cycle_flag), "cycle-flag");
+  if (cycle_flag == 0) return;
+  if (cycle_flag < 0) {
+    report(true);
+    return;
+  }
+  if (_cursor >= (int)_list_mode.size()) return;
+  int timestep = 0;
+  GSBN_CHECK(_glv.geti("simstep", timestep), "simstep");
+  const mode_t& m = _list_mode[_cursor];
+  if (timestep <= m.begin_step) return;
+  if (timestep < m.end_step) {
+    for (auto& p : _pops)
+      op_check(gsbn_check_count(p.H * p.M, p.spike->gpu_data(), p.count, active_stream()),
+               "gsbn_check_count");
+    _updated_flag = true;
+    return;
+  }
+  if (!_updated_flag) {
+    while (_cursor < (int)_list_mode.size() && timestep >= _list_mode[_cursor].end_step) _cursor++;
+    return;
+  }
+  const int w = (int)_window_mode.size();
+  int hoff = 0;
+  for (auto& p : _pops) {
+    op_check(gsbn_check_argmax(p.H, p.M, p.count, _res_idx + (int64_t)w * _total_hcu + hoff,
+                               _res_cnt + (int64_t)w * _total_hcu + hoff, active_stream()),
+             "gsbn_check_argmax");
+    hoff += p.H;
+  }
+  _window_mode.push_back(_cursor);
+  _cursor++;
+}
+
+/* scores windows not yet written (ProcCheck.cpp:193-223) and appends them to
+ * check_result.txt; the summary line (ProcCheck.cpp:227-231) on the final call */
+void ProcCheck::report(bool final_line) {
+  const int nw = (int)_window_mode.size();
+  if (nw > _reported) {
+    hip_check(hipStreamSynchronize(active_stream()), "sync");
+    vector<int32_t> idx((size_t)nw * _total_hcu), cnt((size_t)nw * _total_hcu);
+    hip_check(hipMemcpy(idx.data(), _res_idx, idx.size() * 4, hipMemcpyDeviceToHost), "D2H");
+    hip_check(hipMemcpy(cnt.data(), _res_cnt, cnt.size() * 4, hipMemcpyDeviceToHost), "D2H");
+    std::fstream out(_logfile, std::ios::out | std::ios::app);
+    for (int w = _reported; w < nw; ++w) {
+      const mode_t& m = _list_mode[_window_mode[w]];
+      if ((size_t)(m.lgexp_id + 1) * _lgidx->ld() > _lgidx->size())
+        throw Error(-2, "expected pattern row out of range");
+      const int* exp = _lgidx->cpu_data(m.lgexp_id);
+      const int32_t* wi = idx.data() + (size_t)w * _total_hcu;
+      const int32_t* wc = cnt.data() + (size_t)w * _total_hcu;
+      bool flag = true;
+      for (int i = 0; i < _total_hcu; i++) {
+        if (((wc[i] < _threshold || wi[i] != exp[i]) && exp[i] >= 0) ||
+            (wc[i] > _threshold && exp[i] < 0)) {
+          flag = false;
+          break;
+        }
+      }
+      out << flag << "|[";
+      for (int i = 0; i < _total_hcu; i++) out << exp[i] << ",";
+      out << "]|[";
+      for (int i = 0; i < _total_hcu; i++) out << wi[i] << ",";
+      out << "]|[";
+      for (int i = 0; i < _total_hcu; i++) out << wc[i] << ",";
+      out << "]" << std::endl;
+      _total++;
+      if (flag) _correct++;
+      _flags.push_back(flag ? 1 : 0);
+    }
+    _reported = nw;
+  }
+  if (final_line && !_final_done) {
+    _final_done = true;
+    const double pct = _total ? _correct * 100.0 / (float)_total : 0.0;
+    std::fstream out(_logfile, std::ios::out | std::ios::app);
+    out << "correct pattern: " << _correct << "/" << _total << "(" << pct << "%)" << std::endl;
+  }
+}
+
+void ProcCheck::results(int* correct, int* total, vector<int>* flags) {
+  report(false);
+  *correct = _correct;
+  *total = _total;
+  if (flags) *flags = _flags;
+}
+
+/* lookup used by the C ABI */
+bool check_results(Upd& upd, int* correct, int* total, vector<int>* flags) {
+  for (auto* p : upd.procedures()) {
+    auto* c = dynamic_cast<ProcCheck*>(p);
+    if (c) {
+      c->results(correct, total, flags);
+      return true;
+    }
+  }
+  return false;
+}
+
+}  // namespace gsbn

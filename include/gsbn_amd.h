@@ -1,0 +1,262 @@
+/*
+ * gsbn_amd.h -- C ABI of the MI355X lazy-update BCPNN core.
+ *
+ * Two levels:
+ *
+ * 1. Kernel operations.  Each replaces one device-side piece of the
+ *    reference's ProcUpdLazy (src/gsbn/procedures/ProcUpdLazy/{Pop,Proj}.cu);
+ *    the arguments are plain structs of device pointers, sizes and float
+ *    coefficients; `stream` is a hipStream_t (NULL = default stream).  Every
+ *    function returns 0 on success or a negative gsbn_status / positive
+ *    hipError_t code and never aborts.  Kernels allocate nothing.
+ *
+ * 2. Solver.  A complete configuration-driven simulator (the reference's
+ *    Solver + Upd + procedure plug-ins ProcExtGen, ProcUpdLazy, ProcInitConn,
+ *    ProcExchangeSpike, ProcCheck) behind an opaque handle, reading the
+ *    reference's prototxt / StimRawData / conn-map formats and exposing every
+ *    state vector under the reference's Database names.
+ *
+ * State layout: per-synapse state is one gsbn_syn_t record per synapse, in
+ * the reference's (h*C + k)*M + j order; per-row state one gsbn_rowst_t per
+ * row (h*C + k).  The solver presents both as the reference's separate
+ * vectors (pij_<id>, eij_<id>, ..., pi_<id>, ...).
+ */
+#ifndef GSBN_AMD_H
+#define GSBN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gsbn_status {
+  GSBN_OK = 0,
+  GSBN_ERR_ARG = -1,        /* invalid argument or size */
+  GSBN_ERR_CONFIG = -2,     /* configuration / file error */
+  GSBN_ERR_NAME = -3,       /* unknown vector or global name */
+  GSBN_ERR_STATE = -4,      /* operation not valid in the current state */
+  GSBN_ERR_UNSUPPORTED = -5 /* outside this build's scope */
+};
+
+/* ---- state records (must match gsbn_amd/csrc/gsbn_core.h) ---- */
+typedef struct {
+  float pij, eij, zi2, zj2, wij;
+  int32_t tij;
+  int32_t pad0, pad1;
+} gsbn_syn_rec;
+
+typedef struct {
+  float pi, ei, zi;
+  int32_t ti;
+} gsbn_row_rec;
+
+#define GSBN_HIST 64 /* exec steps kept in the spike queue ring (delays <= 31) */
+#define GSBN_MAX_PROJ_IN 8
+
+/* one projection entering a population, as seen by gsbn_pop_update */
+typedef struct {
+  /* pending support contributions of the previous step's row update:
+   * contrib[(h*C + a)*M + j] for a < pend_cnt[h] (ascending row order) */
+  const float* contrib;
+  int32_t* pend_cnt;
+  int32_t C;
+  /* postsynaptic traces (Proj.cpp:310-363) */
+  float* pj;
+  float* ej;
+  float* zj;
+  float taupdt, ke, kzj, kepsc, kftj, bgain, eps;
+  /* spike queue: hist_list[(e % GSBN_HIST)*Nsrc + i], i < hist_cnt[e % GSBN_HIST],
+   * source units whose si was set at the queue update of exec step e */
+  const int32_t* hist_list;
+  const int32_t* hist_cnt;
+  int32_t Nsrc, dmin, dmax;
+  /* inverse connectivity: for destination HCU h and source unit n the slots
+   * inv_slot[inv_off[h*(Nsrc+1)+n] .. inv_off[h*(Nsrc+1)+n+1]) hold ii == n */
+  const int32_t* inv_off;
+  const int32_t* inv_slot;
+  const int32_t* di;
+  /* outputs: this step's active rows per HCU, ascending (the reference's
+   * .ssi_<id>, Proj.cpp:621-630), and the exec stamp of each active row */
+  int32_t* rows_list;
+  int32_t* rows_cnt;
+  int32_t* act_stamp;
+} gsbn_pop_proj;
+
+/* Fused population step: epsc finalisation, random draws, support, dsup,
+ * WTA soft-max, spike sampling and counters (Pop.cpp:220-387, Pop.cu:17-108),
+ * external-spike override (Pop.cpp:389-409), and for every incoming
+ * projection its postsynaptic update jxx (Proj.cpp:310-363; skipped when
+ * do_jxx == 0), its active-column list (.ssj, Proj.cpp:632-643) and its
+ * active-row list (.ssi via the delay queue, Proj.cpp:621-630, 645-664).
+ * Grid: one workgroup per HCU. */
+typedef struct {
+  int32_t pop_id, H, M, simstep, exec;
+  uint64_t seed;
+  float taumdt, wtagain, maxfqdt, igain, wgain, lgbias, snoise, prn;
+  int32_t wta_pop_max; /* 1: maximum seeded with the population's first unit (Pop.cpp:270) */
+  const float* lginp;  /* N, this population's slice of the current .lginp row */
+  const float* wmask;  /* H, this population's slice of the current .wmask row */
+  float* dsup;
+  float* act;
+  int8_t* spike;
+  int32_t* counter;
+  float* rnd_u;
+  float* rnd_n;
+  float* epsc; /* [nproj_in][N] */
+  float* bj;   /* [nproj_in][N] */
+  float* scratch_dsup0; /* device float, used when wta_pop_max (filled by this call) */
+  const int32_t* ext_off; /* external spikes CSR by simstep, or NULL */
+  const int32_t* ext_units;
+  int32_t ext_nsteps;
+  int32_t* spk_list; /* [H][M] spiking units of each HCU, ascending */
+  int32_t* spk_cnt;  /* [H] */
+  int32_t nproj_in, do_jxx;
+  int32_t max_slots; /* largest C among the incoming projections (sizes the row bitmap) */
+  const gsbn_pop_proj* projs; /* device array, index = position of the projection in the population */
+  int64_t* work; /* optional device counters [2*nproj_in]: row and column synapse updates */
+} gsbn_pop_args;
+int gsbn_pop_update(const gsbn_pop_args* a, void* stream);
+
+/* Apply pending support contributions to epsc without a population update
+ * (used before host reads of epsc_<pop>). Grid: one workgroup per HCU. */
+int gsbn_pop_flush(const gsbn_pop_args* a, void* stream);
+
+/* Full catch-up on a learning-rate change (Proj.cpp:209-308, 538-588):
+ * every row and synapse to `target` (= simstep-1) with the table of the old
+ * learning rate; Wij recomputed when kp != 0. */
+typedef struct {
+  int32_t H, M, C, target;
+  float kp, wgain, eps, eps2;
+  const float* table;
+  int32_t table_len;
+  gsbn_syn_rec* syn;
+  gsbn_row_rec* rowst;
+  const float* pj;
+} gsbn_all_args;
+int gsbn_proj_all(const gsbn_all_args* a, void* stream);
+
+/* Postsynaptic Euler update and bias (Proj.cpp:310-363), standalone form. */
+typedef struct {
+  int32_t N;
+  const int8_t* sj;
+  float *pj, *ej, *zj, *bj, *epsc;
+  float kp, ke, kzj, kepsc, kftj, bgain, eps;
+} gsbn_jxx_args;
+int gsbn_proj_jxx(const gsbn_jxx_args* a, void* stream);
+
+/* Row and column lazy updates of one step (Proj.cpp:365-536, 667-760):
+ * active rows catch up with kfti on zi/zi2, Wij when kp != 0, and write their
+ * support contribution; columns of spiking postsynaptic units catch up with
+ * kftj on zj2 (slots with ii < 0 skipped).  A synapse in both an active row
+ * and an active column gets the row update followed by the column increment,
+ * as in the reference's row-then-column order. */
+typedef struct {
+  int32_t H, M, C, simstep, exec;
+  float kp, kfti, kftj, wgain, eps, eps2;
+  const float* table;
+  int32_t table_len;
+  gsbn_syn_rec* syn;
+  gsbn_row_rec* rowst;
+  const float* pj;
+  const int8_t* sj;
+  const int32_t* ii;
+  const int32_t* rows_list;
+  const int32_t* rows_cnt;
+  const int32_t* spk_list;
+  const int32_t* spk_cnt;
+  const int32_t* act_stamp;
+  float* contrib;
+  int32_t grid; /* workgroups to launch (0: default) */
+} gsbn_rowcol_args;
+int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream);
+
+/* Delay-queue push (Proj.cpp:645-664): record the source units with si > 0
+ * for exec step `exec`. */
+typedef struct {
+  int32_t Nsrc, exec;
+  const int8_t* si;
+  int32_t* hist_list;
+  int32_t* hist_cnt;
+} gsbn_que_args;
+int gsbn_proj_que(const gsbn_que_args* a, void* stream);
+
+/* ProcCheck counting (ProcCheck/Pop.cpp:20-28) and per-HCU first strict
+ * argmax with counter reset (ProcCheck/Pop.cpp:35-50). */
+int gsbn_check_count(int32_t n, const int8_t* spike, int32_t* count, void* stream);
+int gsbn_check_argmax(int32_t H, int32_t M, int32_t* count, int32_t* out_idx, int32_t* out_cnt,
+                      void* stream);
+
+/* field gather/scatter between the record arrays and plain vectors
+ * field: 0 pij 1 eij 2 zi2 3 zj2 4 wij 5 tij (syn); 0 pi 1 ei 2 zi 3 ti (row) */
+int gsbn_syn_field(gsbn_syn_rec* syn, int64_t n, int field, void* buf, int to_records,
+                   void* stream);
+int gsbn_row_field(gsbn_row_rec* row, int64_t n, int field, void* buf, int to_records,
+                   void* stream);
+
+/* fill record / int arrays with one value (state initialisation) */
+int gsbn_syn_fill(gsbn_syn_rec* syn, int64_t n, gsbn_syn_rec v, void* stream);
+int gsbn_row_fill(gsbn_row_rec* row, int64_t n, gsbn_row_rec v, void* stream);
+int gsbn_i32_fill(int32_t* p, int64_t n, int32_t v, void* stream);
+
+/* Coefficient table entry (host function): GSBN_COEF_STRIDE = 12 floats
+ * Ezi Ezj Ee Ep Cez_ij Cpe Cpz_ij Cez_i Cpz_i 0 0 0 for an elapsed gap. */
+void gsbn_coeff_host_entry(double kzi, double kzj, double ke, double kp, int64_t gap,
+                           float* out12);
+int64_t gsbn_coeff_host_saturation(double kzi, double kzj, double ke, double kp,
+                                   int64_t max_len);
+
+/* deterministic host copies of the device math, for tests */
+float gsbn_host_expf(float x);
+float gsbn_host_logf(float x);
+void gsbn_host_draw(uint64_t seed, uint32_t pop, uint32_t step, uint32_t unit, float* u,
+                    float* z);
+/* evaluate the device math on the GPU for n inputs (tests) */
+int gsbn_device_math(int op, int64_t n, const void* in_host, void* out_host);
+
+/* ---- solver level ---- */
+typedef struct gsbn_solver gsbn_solver;
+
+/* Solver(n_path, s_path) (Solver.cpp:10-65); s_path must be NULL or empty
+ * (snapshot resume is outside this build). NULL on error, message in
+ * gsbn_last_error(). */
+gsbn_solver* gsbn_solver_new(const char* net_path, const char* snapshot_path);
+void gsbn_solver_free(gsbn_solver* s);
+/* run Upd::update up to nsteps times, stopping when cycle-flag < 0;
+ * returns the number of updates performed or an error code */
+int64_t gsbn_solver_step(gsbn_solver* s, int64_t nsteps);
+/* Solver::run: until cycle-flag < 0 */
+int64_t gsbn_solver_run(gsbn_solver* s);
+int gsbn_solver_sync(gsbn_solver* s);
+/* vector access by Database name; dtype codes 0 i8 1 i16 2 i32 3 i64 4 f16 5 f32 6 f64 */
+int64_t gsbn_solver_vector_info(gsbn_solver* s, const char* name, int* dtype);
+int64_t gsbn_solver_vector_read(gsbn_solver* s, const char* name, void* out, int64_t cap);
+int gsbn_solver_vector_write(gsbn_solver* s, const char* name, const void* in, int64_t count);
+/* GlobalVar access: type 'i', 'f', 'b' */
+int gsbn_solver_global(gsbn_solver* s, const char* key, char type, void* out);
+/* ProcCheck results: correct / total windows, per-window flags */
+int gsbn_solver_check(gsbn_solver* s, int32_t* correct, int32_t* total, int32_t* flags,
+                      int32_t cap);
+/* counters: [0] row synapse updates [1] column synapse updates [2] sweep
+ * synapse updates [3] executed steps [4] sweeps */
+int gsbn_solver_work(gsbn_solver* s, int64_t* out5);
+/* per-kernel timing with HIP events on the solver stream.  enable > 0 is a
+ * bit mask of the kinds to time (0 pop, 1 rowcol, 2 all, 3 jxx, 4 que, 5
+ * check), 0 stops timing, < 0 also clears the totals; when launches6/ms6 are
+ * given the totals {launches, total_ms} per kind are written first. */
+int gsbn_solver_timing(gsbn_solver* s, int enable, int64_t* launches6, double* ms6);
+
+/* parse a SolverParam prototxt (and its stimulus file) without a device:
+ * out4 = {populations, projections, mode_params, proc_params} */
+int gsbn_config_check(const char* net_path, int32_t* out4);
+/* decode a StimRawData file: dims4 = {data_rows, data_cols, mask_rows, mask_cols} */
+int gsbn_stim_read(const char* path, int32_t* dims4, uint32_t* data, int64_t data_cap, float* mask,
+                   int64_t mask_cap);
+
+const char* gsbn_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSBN_AMD_H */

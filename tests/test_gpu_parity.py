@@ -1,0 +1,53 @@
+"""GPU parity: the HIP path against the CPU oracle, bit for bit (-m gpu)."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gsbn_amd import configs, device_math, host_expf, host_logf
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_math_bitwise():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-120, 90, 200000).astype(np.float32),
+                        np.array([0, -0.0, 1, -1, 88.7, -103.9, 1e-40, np.inf, -np.inf, np.nan],
+                                 np.float32)])
+    ge = device_math(0, x.view(np.uint32)).view(np.float32)
+    he = np.array([host_expf(v) for v in x[:20000]], np.float32)
+    assert (ge[:20000].view(np.uint32) == he.view(np.uint32)).all()
+    y = np.abs(x[np.isfinite(x)]) + np.float32(1e-30)
+    gl = device_math(1, y.view(np.uint32)).view(np.float32)
+    hl = np.array([host_logf(v) for v in y[:20000]], np.float32)
+    assert (gl[:20000].view(np.uint32) == hl.view(np.uint32)).all()
+    # IEEE division must round identically (the WTA normalisation and Wij)
+    ab = rng.uniform(1e-6, 10, (100000, 2)).astype(np.float32)
+    q = device_math(3, ab.reshape(-1).view(np.uint32)).view(np.float32)
+    assert (q.view(np.uint32) == (ab[:, 0] / ab[:, 1]).view(np.uint32)).all()
+
+
+def test_device_draw_matches_oracle():
+    rows = []
+    for seed in (0, 12345678901):
+        for step in (0, 1, 77, 100000):
+            for unit in (0, 1, 63, 9999):
+                rows.append([seed & 0xffffffff, seed >> 32, step, unit])
+    out = device_math(2, np.array(rows, np.uint32).reshape(-1)).view(np.float32).reshape(-1, 2)
+    for (lo, hi, step, unit), (u, z) in zip(rows, out):
+        ou, oz = O.draw(lo | (hi << 32), 0, step, unit)
+        assert np.float32(ou) == u and np.float32(oz) == z
+
+
+def test_config0_full_run_bitwise(tmp_path):
+    spec = configs.config0()
+    gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path)
+    assert div is None, "spike rasters diverge at step %s" % (div,)
+    bad = helpers.compare_state(gpu, ora, spec)
+    assert not bad, bad
+    c, t, flags = gpu.check()
+    assert (c, t) == (ora.check.correct, ora.check.total)
+    assert list(flags) == [int(r["ok"]) for r in ora.check.results]
+    gpu.close()
