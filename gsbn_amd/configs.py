@@ -95,6 +95,11 @@ class NetSpec:
     check_threshold: int = 0
     ext_spikes: Optional[Dict[int, Dict[int, List[int]]]] = None  # pop -> step -> units
     name: str = "net"
+    exchange_first: bool = False  # list ProcExchangeSpike before ProcUpdLazy
+    conn_text: Optional[str] = None  # verbatim connection-map text (overrides conn)
+    # per projection (ii, di) arrays of shape [H*C], written through the
+    # Database API instead of a connection-map file (no ProcInitConn)
+    conn_arrays: Optional[List[Tuple[np.ndarray, np.ndarray]]] = None
 
     @property
     def total_hcu(self) -> int:
@@ -235,6 +240,8 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
     L.append("}")
     L.append('rec_param { directory: "%s" }' % rec_dir)
     L.append('proc_param { name: "ProcExtGen" }')
+    if spec.exchange_first:
+        L.append('proc_param { name: "ProcExchangeSpike" }')
     L.append("proc_param {")
     L.append('  name: "ProcUpdLazy"')
     L.append('  argi { key: "seed-lo" val: %d }' % (spec.seed & 0x7FFFFFFF))
@@ -243,11 +250,13 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
     if ext_spike_path:
         L.append('  args { key: "external spike" val: "%s" }' % ext_spike_path)
     L.append("}")
-    L.append("proc_param {")
-    L.append('  name: "ProcInitConn"')
-    L.append('  args { key: "conn-map-file" val: "%s" }' % conn_path)
-    L.append("}")
-    L.append('proc_param { name: "ProcExchangeSpike" }')
+    if spec.conn_arrays is None:
+        L.append("proc_param {")
+        L.append('  name: "ProcInitConn"')
+        L.append('  args { key: "conn-map-file" val: "%s" }' % conn_path)
+        L.append("}")
+    if not spec.exchange_first:
+        L.append('proc_param { name: "ProcExchangeSpike" }')
     L.append("proc_param {")
     L.append('  name: "ProcCheck"')
     L.append('  argi { key: "threashold" val: %d }' % spec.check_threshold)
@@ -256,6 +265,8 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
 
 
 def conn_map_text(spec: NetSpec) -> str:
+    if spec.conn_text is not None:
+        return spec.conn_text
     lines = []
     for pid, rows in enumerate(spec.conn):
         for dest_hcu, pairs in rows:
@@ -278,8 +289,9 @@ def write(spec: NetSpec, directory: str) -> str:
     rec_dir = os.path.join(directory, spec.name + "_rec")
     with open(stim_path, "wb") as f:
         f.write(encode_stim(spec.stim, spec.mask))
-    with open(conn_path, "w") as f:
-        f.write(conn_map_text(spec))
+    if spec.conn_arrays is None:
+        with open(conn_path, "w") as f:
+            f.write(conn_map_text(spec))
     ext_path = None
     if spec.ext_spikes:
         ext_path = os.path.join(directory, spec.name + "_ext.csv")
@@ -292,6 +304,19 @@ def write(spec: NetSpec, directory: str) -> str:
     with open(path, "w") as f:
         f.write(prototxt(spec, stim_path, conn_path, rec_dir, ext_path))
     return path
+
+
+def parse_conn_map(text: str, nproj: int = 1):
+    """Connection-map text -> per projection [(dest_hcu, [(src, delay), ...])]."""
+    out: List[List[Tuple[int, List[Tuple[int, int]]]]] = [[] for _ in range(nproj)]
+    for line in text.splitlines():
+        v = line.strip().split(",")
+        if len(v) <= 2:
+            continue
+        pid, h = int(v[0]), int(v[1])
+        pairs = [(int(v[i]), int(v[i + 1])) for i in range(2, len(v) - 1, 2)]
+        out[pid].append((h, pairs))
+    return out
 
 
 def dense_conn(hcu: int, mcu: int, delay: int = 1, self_hcu: bool = False):
@@ -340,6 +365,18 @@ def recall_net(hcu: int, mcu: int, n_pat: int, train: int, quiet: int, recall: i
                 masks[i, h] = 1.0
     stim = np.concatenate([pats, quiet_row, cues], axis=0)
     mask = np.concatenate([np.zeros((1, hcu), dtype=np.float32), masks], axis=0)
+    modes = recall_schedule(n_pat, train, quiet, recall, reset, reset_prn, dt)
+    pop = PopSpec(hcu=hcu, mcu=mcu, wtagain=wtagain, wgain=pop_wgain, snoise=snoise)
+    proj = ProjSpec(src=0, dest=0, tauzi=tauzi, tauzj=tauzj, taue=taue, taup=taup,
+                    tauepsc=tauepsc, slot_num=float((hcu - 1) * mcu))
+    return NetSpec(dt=dt, pops=[pop], projs=[proj], modes=modes, stim=stim, mask=mask,
+                   conn=[dense_conn(hcu, mcu, delay)], seed=seed, name=name)
+
+
+def recall_schedule(n_pat: int, train: int, quiet: int, recall: int, reset: int = 0,
+                    reset_prn: float = 1e-30, dt: float = 0.001) -> List[ModeSpec]:
+    """Stimulus rows: 0..n_pat-1 patterns, n_pat quiet, n_pat+1.. cues; mask
+    rows: 0 training, 1.. one per cue."""
     modes: List[ModeSpec] = []
     t = 0
     for i in range(n_pat):
@@ -368,11 +405,7 @@ def recall_net(hcu: int, mcu: int, n_pat: int, train: int, quiet: int, recall: i
                                   begin_lgidx_id=n_pat + 1 + i, begin_wmask_id=1 + i,
                                   begin_lgexp_id=i, prn=0.0))
             t += recall
-    pop = PopSpec(hcu=hcu, mcu=mcu, wtagain=wtagain, wgain=pop_wgain, snoise=snoise)
-    proj = ProjSpec(src=0, dest=0, tauzi=tauzi, tauzj=tauzj, taue=taue, taup=taup,
-                    tauepsc=tauepsc, slot_num=float((hcu - 1) * mcu))
-    return NetSpec(dt=dt, pops=[pop], projs=[proj], modes=modes, stim=stim, mask=mask,
-                   conn=[dense_conn(hcu, mcu, delay)], seed=seed, name=name)
+    return modes
 
 
 def config0(seed: int = 0) -> NetSpec:
@@ -386,3 +419,70 @@ def config1(seed: int = 0) -> NetSpec:
     steps, 100 cues with 30% of the HCUs clamped, 50 recall steps each after a
     50-step reset."""
     return recall_net(100, 100, 100, 50, 10, 50, 0.3, seed=seed, reset=50, name="cfg1")
+
+
+def apply_conn_arrays(target, spec: NetSpec):
+    """Write ii_<id>/di_<id> of every projection (a gsbn_amd.Solver or an
+    oracle network: anything with write(name, array))."""
+    for pid, (ii, di) in enumerate(spec.conn_arrays or []):
+        target.write("ii_%d" % pid, ii)
+        target.write("di_%d" % pid, di)
+
+
+def sparse_conn_arrays(rng: np.random.Generator, hcu: int, mcu: int, p: float, delay: int = 1):
+    """Bernoulli(p) recurrent connectivity without self-HCU connections: slot k
+    of HCU h holds the k-th selected source unit (ascending), as ProcInitConn
+    fills slots in file order.  Returns (ii, di, C)."""
+    n = hcu * mcu
+    src = np.arange(n)
+    masks = rng.random((hcu, n)) < p
+    for h in range(hcu):
+        masks[h, h * mcu:(h + 1) * mcu] = False
+    counts = masks.sum(axis=1)
+    C = max(1, int(counts.max()))
+    ii = np.full((hcu, C), -1, np.int32)
+    di = np.zeros((hcu, C), np.int32)
+    for h in range(hcu):
+        sel = src[masks[h]]
+        ii[h, :len(sel)] = sel
+        di[h, :len(sel)] = delay
+    return ii.reshape(-1), di.reshape(-1), C
+
+
+def batched_nets(n_net: int = 512, n_pat: int = 20, train: int = 50, quiet: int = 10,
+                 recall: int = 50, reset: int = 50, clamp_frac: float = 0.5, *,
+                 hcu_range=(8, 64), mcu_range=(16, 128), probs=(0.1, 0.5, 0.9), seed: int = 0,
+                 snoise: float = 0.1, wtagain: float = 4.0, pop_wgain: float = 8.0,
+                 dt: float = 0.001, name: str = "batch") -> NetSpec:
+    """n_net independent networks, each its own population and sparse
+    recurrent projection, stepped in lock-step by one schedule.  Network i has
+    H ~ U{hcu_range}, M ~ U{mcu_range}, connection probability from `probs`,
+    its own random streams (population id) and its own patterns."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    pops, projs, conns, stim_cols, mask_cols = [], [], [], [], []
+    for i in range(n_net):
+        H = int(rng.integers(hcu_range[0], hcu_range[1] + 1))
+        M = int(rng.integers(mcu_range[0], mcu_range[1] + 1))
+        p = float(probs[int(rng.integers(0, len(probs)))])
+        ii, di, C = sparse_conn_arrays(rng, H, M, p)
+        pops.append(PopSpec(hcu=H, mcu=M, wtagain=wtagain, wgain=pop_wgain, snoise=snoise))
+        projs.append(ProjSpec(src=i, dest=i, tauzi=0.01, tauzj=0.01, taue=0.02, taup=1.0,
+                              slot_num=float(C)))
+        conns.append((ii, di))
+        pats = random_patterns(rng, n_pat, H, M)
+        cues = pats.copy()
+        masks = np.zeros((n_pat, H), np.float32)
+        nc = int(round(clamp_frac * H))
+        for k in range(n_pat):
+            clamped = set(int(v) for v in rng.permutation(H)[:nc])
+            for h in range(H):
+                if h not in clamped:
+                    cues[k, h] = UNCLAMPED
+                    masks[k, h] = 1.0
+        stim_cols.append(np.concatenate([pats, np.full((1, H), UNCLAMPED, np.uint32), cues]))
+        mask_cols.append(np.concatenate([np.zeros((1, H), np.float32), masks]))
+    stim = np.concatenate(stim_cols, axis=1)
+    mask = np.concatenate(mask_cols, axis=1)
+    modes = recall_schedule(n_pat, train, quiet, recall, reset, dt=dt)
+    return NetSpec(dt=dt, pops=pops, projs=projs, modes=modes, stim=stim, mask=mask,
+                   conn=[[] for _ in projs], conn_arrays=conns, seed=seed, name=name)

@@ -117,6 +117,12 @@ class VecBase {
   std::function<void()> before_host_read; /* e.g. flush pending device work */
   uint64_t host_version = 0;              /* bumped on every mutable host access */
   Status status = SYNCED;
+  /* process-wide count of host-side accesses that moved or invalidated data;
+   * procedures compare it between steps to skip re-validating their vectors */
+  static uint64_t& host_epoch() {
+    static uint64_t e = 0;
+    return e;
+  }
 };
 
 template <typename T>
@@ -172,6 +178,7 @@ class SyncVector : public VecBase {
   void mark_host() {
     status = HOST_NEWER;
     ++host_version;
+    ++host_epoch();
   }
 
   void* host_ptr_sync() override {
@@ -189,6 +196,7 @@ class SyncVector : public VecBase {
 
   void to_host() {
     if (status != DEVICE_NEWER) return;
+    ++host_epoch();
     if (before_host_read) before_host_read();
     if (view) {
       h_.resize(n_);

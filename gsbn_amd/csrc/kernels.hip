@@ -113,7 +113,7 @@ __global__ void pop_dsup0_kernel(gsbn_pop_args a) {
   a.scratch_dsup0[0] = unit_dsup(a, 0, 0, efin, &u, &xi);
 }
 
-__global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
+__device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   extern __shared__ uint32_t sh_bitmap[];
   __shared__ float sh_f[POP_MAX_THREADS];
   __shared__ int sh_ext[POP_MAX_THREADS];
@@ -121,7 +121,6 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
   __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
   __shared__ float sh_vsum;
 
-  const int h = blockIdx.x;
   const int j = threadIdx.x;
   const int M = a.M;
   const bool live = j < M;
@@ -215,6 +214,11 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
     int pos = block_excl_scan(live ? spk : 0, sh_scan, &spk_tot);
     if (live && spk) a.spk_list[h * M + pos] = j;
     if (j == 0) a.spk_cnt[h] = spk_tot;
+    if (a.hist_out_list) {
+      const int slot = (a.exec + 1) % GSBN_HIST;
+      if (live && spk) a.hist_out_list[((int64_t)slot * a.H + h) * M + pos] = j;
+      if (j == 0) a.hist_out_cnt[(int64_t)slot * a.H + h] = spk_tot;
+    }
   }
 
   /* 6. postsynaptic traces and bias for every incoming projection
@@ -245,18 +249,21 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
     const int W = (C + 31) >> 5;
     for (int w = j; w < W; w += blockDim.x) sh_bitmap[w] = 0u;
     __syncthreads();
+    const int32_t* off = p.inv_off + (int64_t)h * (p.Nsrc + 1);
     for (int d = p.dmin; d <= p.dmax; ++d) {
       const int e = a.exec - d - 1;
       if (e < 0) continue;
       const int slot = e % GSBN_HIST;
-      const int cnt = p.hist_cnt[slot];
-      const int32_t* lst = p.hist_list + (int64_t)slot * p.Nsrc;
-      const int32_t* off = p.inv_off + (int64_t)h * (p.Nsrc + 1);
-      for (int t = j; t < cnt; t += blockDim.x) {
-        const int n = lst[t];
-        for (int q = off[n]; q < off[n + 1]; ++q) {
-          const int k = p.inv_slot[q];
-          if (p.di[(int64_t)h * C + k] == d) atomicOr(&sh_bitmap[k >> 5], 1u << (k & 31));
+      const int32_t* cnts = p.hist_cnt + (int64_t)slot * p.Hsrc;
+      const int32_t* lsts = p.hist_list + (int64_t)slot * p.Hsrc * p.Msrc;
+      for (int hs = j; hs < p.Hsrc; hs += blockDim.x) {
+        const int cnt = cnts[hs];
+        for (int t = 0; t < cnt; ++t) {
+          const int n = hs * p.Msrc + lsts[(int64_t)hs * p.Msrc + t];
+          for (int q = off[n]; q < off[n + 1]; ++q) {
+            const int k = p.inv_slot[q];
+            if (p.di[(int64_t)h * C + k] == d) atomicOr(&sh_bitmap[k >> 5], 1u << (k & 31));
+          }
         }
       }
     }
@@ -293,6 +300,41 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
   }
 }
 
+__global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
+  pop_body(a, blockIdx.x);
+}
+
+/* per-step fields shared by every population / projection of a batch */
+__device__ __forceinline__ void apply_step(gsbn_pop_args& a, const gsbn_step_args& st) {
+  a.simstep = st.simstep;
+  a.exec = st.exec;
+  a.prn = st.prn;
+  a.do_jxx = st.do_jxx;
+  a.lginp += st.lginp_offset;
+  a.wmask += st.wmask_offset;
+}
+
+__global__ void __launch_bounds__(POP_MAX_THREADS)
+    pop_kernel_batch(const gsbn_pop_args* __restrict__ arr, const int32_t* __restrict__ wg_pop,
+                     const int32_t* __restrict__ wg_hcu, gsbn_step_args st) {
+  gsbn_pop_args a = arr[wg_pop[blockIdx.x]];
+  apply_step(a, st);
+  pop_body(a, wg_hcu[blockIdx.x]);
+}
+
+__global__ void pop_dsup0_kernel_batch(const gsbn_pop_args* __restrict__ arr, int npop,
+                                       gsbn_step_args st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npop) return;
+  gsbn_pop_args a = arr[i];
+  if (!a.wta_pop_max) return;
+  apply_step(a, st);
+  float efin[GSBN_MAX_PROJ_IN];
+  for (int m = 0; m < a.nproj_in; ++m) efin[m] = finalized_epsc(a, a.projs[m], m, 0, 0, a.projs[m].pend_cnt[0]);
+  float u, xi;
+  a.scratch_dsup0[0] = unit_dsup(a, 0, 0, efin, &u, &xi);
+}
+
 __global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_args a) {
   __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
   const int h = blockIdx.x, j = threadIdx.x;
@@ -307,150 +349,123 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_arg
   if (j < a.nproj_in) a.projs[j].pend_cnt[h] = 0;
 }
 
-/* largest h in [0, n-1] with off[h] <= g (off non-decreasing, off[0] = 0) */
-__device__ inline int find_owner(const int* off, int n, int g) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= g)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-__global__ void __launch_bounds__(RC_THREADS) rowcol_kernel(gsbn_rowcol_args a) {
-  extern __shared__ int sh_off[]; /* row_off[H+1], col_off[H+1] */
-  __shared__ int sh_scan[RC_THREADS / 64 + 1];
+__device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const int h, const int part) {
   __shared__ float sh_pi[RC_THREADS];
-  __shared__ int sh_h[RC_THREADS];
   __shared__ int sh_a[RC_THREADS];
   __shared__ int sh_r[RC_THREADS];
 
-  const int H = a.H, M = a.M, C = a.C;
+  const int P = a.parts;
+  const int M = a.M, C = a.C;
   const int tid = threadIdx.x;
-  int* row_off = sh_off;
-  int* col_off = sh_off + (H + 1);
-  const int nblk = (C + RC_THREADS - 1) / RC_THREADS;
-
-  int run_r = 0, run_c = 0;
-  for (int base = 0; base < H; base += RC_THREADS) {
-    const int hh = base + tid;
-    const int vr = hh < H ? a.rows_cnt[hh] : 0;
-    const int vc = hh < H ? a.spk_cnt[hh] * nblk : 0;
-    int tr, tc;
-    const int er = block_excl_scan(vr, sh_scan, &tr);
-    const int ec = block_excl_scan(vc, sh_scan, &tc);
-    if (hh < H) {
-      row_off[hh] = run_r + er;
-      col_off[hh] = run_c + ec;
-    }
-    run_r += tr;
-    run_c += tc;
-  }
-  if (tid == 0) {
-    row_off[H] = run_r;
-    col_off[H] = run_c;
-  }
-  __syncthreads();
-
-  const int A = run_r, Q = run_c;
-  const int rpt = M >= RC_THREADS ? 1 : RC_THREADS / M;
-  const int ntr = (A + rpt - 1) / rpt;
-  const int ntask = ntr + Q;
   const int t = a.simstep;
 
-  for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
-    if (task < ntr) {
-      const int g0 = task * rpt;
-      const int nr = (A - g0 < rpt) ? A - g0 : rpt;
-      if (tid < nr) {
-        const int g = g0 + tid;
-        const int h = find_owner(row_off, H, g);
-        const int aa = g - row_off[h];
-        const int k = a.rows_list[(int64_t)h * C + aa];
-        const int r = h * C + k;
-        gsbn_row_rec st = a.rowst[r];
-        const int D = t - st.ti;
-        if (D <= 0) {
-          st.zi += a.kfti;
-          st.ti = t;
+  /* rows part, part+P, ... of this HCU's ascending active-row list */
+  const int nrows = a.rows_cnt[h];
+  const int mine = nrows > part ? (nrows - part + P - 1) / P : 0;
+  const int rpt = M >= RC_THREADS ? 1 : RC_THREADS / M;
+  for (int base = 0; base < mine; base += rpt) {
+    const int nr = (mine - base < rpt) ? mine - base : rpt;
+    if (tid < nr) {
+      const int aa = part + P * (base + tid);
+      const int k = a.rows_list[(int64_t)h * C + aa];
+      const int r = h * C + k;
+      gsbn_row_rec st = a.rowst[r];
+      const int D = t - st.ti;
+      if (D <= 0) {
+        st.zi += a.kfti;
+        st.ti = t;
+      } else {
+        const float* c = gsbn_coef(a.table, a.table_len, D);
+        gsbn_row_catchup(c, &st.pi, &st.ei, &st.zi);
+        st.zi = st.zi + a.kfti;
+        st.ti = t;
+      }
+      a.rowst[r] = st;
+      sh_pi[tid] = st.pi;
+      sh_a[tid] = aa;
+      sh_r[tid] = r;
+    }
+    __syncthreads();
+    for (int f = tid; f < nr * M; f += RC_THREADS) {
+      const int rr = f / M;
+      const int j = f - rr * M;
+      const int64_t sidx = (int64_t)sh_r[rr] * M + j;
+      gsbn_syn_rec s = a.syn[sidx];
+      float contrib;
+      const int D = t - s.tij;
+      if (D <= 0) {
+        s.zi2 += a.kfti;
+        s.tij = t;
+        contrib = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
+      } else {
+        const float* c = gsbn_coef(a.table, a.table_len, D);
+        gsbn_syn_catchup(c, &s.pij, &s.eij, &s.zi2, &s.zj2);
+        s.zi2 = s.zi2 + a.kfti;
+        s.tij = t;
+        float w;
+        if (a.kp != 0.0f) {
+          w = gsbn_wij(s.pij, sh_pi[rr], a.pj[h * M + j], a.wgain, a.eps, a.eps2);
+          s.wij = w;
         } else {
-          const float* c = gsbn_coef(a.table, a.table_len, D);
-          gsbn_row_catchup(c, &st.pi, &st.ei, &st.zi);
-          st.zi = st.zi + a.kfti;
-          st.ti = t;
+          w = s.wij;
         }
-        a.rowst[r] = st;
-        sh_pi[tid] = st.pi;
-        sh_h[tid] = h;
-        sh_a[tid] = aa;
-        sh_r[tid] = r;
+        contrib = w;
       }
-      __syncthreads();
-      for (int f = tid; f < nr * M; f += RC_THREADS) {
-        const int rr = f / M;
-        const int j = f - rr * M;
-        const int h = sh_h[rr];
-        const int64_t sidx = (int64_t)sh_r[rr] * M + j;
-        gsbn_syn_rec s = a.syn[sidx];
-        float contrib;
-        const int D = t - s.tij;
-        if (D <= 0) {
-          s.zi2 += a.kfti;
-          s.tij = t;
-          contrib = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
-        } else {
-          const float* c = gsbn_coef(a.table, a.table_len, D);
-          gsbn_syn_catchup(c, &s.pij, &s.eij, &s.zi2, &s.zj2);
-          s.zi2 = s.zi2 + a.kfti;
-          s.tij = t;
-          float w;
-          if (a.kp != 0.0f) {
-            w = gsbn_wij(s.pij, sh_pi[rr], a.pj[h * M + j], a.wgain, a.eps, a.eps2);
-            s.wij = w;
-          } else {
-            w = s.wij;
-          }
-          contrib = w;
-        }
-        /* column increment of a spiking postsynaptic unit after the row
-         * update (the reference's col pass sees D == 0 here) */
-        if (a.sj[h * M + j] > 0) s.zj2 += a.kftj;
-        a.syn[sidx] = s;
-        a.contrib[((int64_t)h * C + sh_a[rr]) * M + j] = contrib;
+      /* column increment of a spiking postsynaptic unit after the row
+       * update (the reference's col pass sees D == 0 here) */
+      if (a.sj[h * M + j] > 0) s.zj2 += a.kftj;
+      a.syn[sidx] = s;
+      a.contrib[((int64_t)h * C + sh_a[rr]) * M + j] = contrib;
+    }
+    __syncthreads();
+  }
+
+  /* slice `part` of the slots of every spiking unit of this HCU */
+  const int nspk = a.spk_cnt[h];
+  if (nspk == 0) return;
+  const int slice = (C + P - 1) / P;
+  const int k0 = part * slice;
+  const int k1 = (k0 + slice < C) ? k0 + slice : C;
+  for (int s_i = 0; s_i < nspk; ++s_i) {
+    const int j = a.spk_list[h * M + s_i];
+    for (int k = k0 + tid; k < k1; k += RC_THREADS) {
+      const int r = h * C + k;
+      if (a.ii[r] < 0 || a.act_stamp[r] == a.exec) continue;
+      gsbn_syn_rec* sp = a.syn + (int64_t)r * M + j;
+      float4 v = *reinterpret_cast<float4*>(sp);
+      const int tij = sp->tij;
+      float pij = v.x, eij = v.y, zi2 = v.z, zj2 = v.w;
+      const int D = t - tij;
+      if (D <= 0) {
+        zj2 += a.kftj;
+      } else {
+        const float* c = gsbn_coef(a.table, a.table_len, D);
+        gsbn_syn_catchup(c, &pij, &eij, &zi2, &zj2);
+        zj2 = zj2 + a.kftj;
       }
-      __syncthreads();
-    } else {
-      const int q = task - ntr;
-      const int h = find_owner(col_off, H, q);
-      const int local = q - col_off[h];
-      const int s_i = local / nblk;
-      const int b = local - s_i * nblk;
-      const int j = a.spk_list[h * M + s_i];
-      const int k = b * RC_THREADS + tid;
-      if (k < C) {
-        const int r = h * C + k;
-        if (a.ii[r] >= 0 && a.act_stamp[r] != a.exec) {
-          gsbn_syn_rec* sp = a.syn + (int64_t)r * M + j;
-          float4 v = *reinterpret_cast<float4*>(sp);
-          int tij = sp->tij;
-          float pij = v.x, eij = v.y, zi2 = v.z, zj2 = v.w;
-          const int D = t - tij;
-          if (D <= 0) {
-            zj2 += a.kftj;
-          } else {
-            const float* c = gsbn_coef(a.table, a.table_len, D);
-            gsbn_syn_catchup(c, &pij, &eij, &zi2, &zj2);
-            zj2 = zj2 + a.kftj;
-          }
-          *reinterpret_cast<float4*>(sp) = make_float4(pij, eij, zi2, zj2);
-          sp->tij = t;
-        }
-      }
+      *reinterpret_cast<float4*>(sp) = make_float4(pij, eij, zi2, zj2);
+      sp->tij = t;
     }
   }
+}
+
+__global__ void __launch_bounds__(RC_THREADS) rowcol_kernel(gsbn_rowcol_args a) {
+  const int h = blockIdx.x / a.parts;
+  rowcol_body(a, h, blockIdx.x - h * a.parts);
+}
+
+__global__ void __launch_bounds__(RC_THREADS)
+    rowcol_kernel_batch(const gsbn_rowcol_args* __restrict__ arr, const int32_t* __restrict__ wg_proj,
+                        const int32_t* __restrict__ wg_hcu, int parts, gsbn_step_args st) {
+  const int w = blockIdx.x / parts;
+  gsbn_rowcol_args a = arr[wg_proj[w]];
+  a.simstep = st.simstep;
+  a.exec = st.exec;
+  a.kp = a.taupdt * st.prn;
+  a.table = a.tables[st.kp_slot];
+  a.table_len = a.table_lens[st.kp_slot];
+  a.parts = parts;
+  rowcol_body(a, wg_hcu[w], blockIdx.x - w * parts);
 }
 
 __global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
@@ -516,20 +531,22 @@ __global__ void jxx_kernel(gsbn_jxx_args a) {
   }
 }
 
+/* one workgroup per source HCU; ascending MCU order within the HCU */
 __global__ void __launch_bounds__(1024) que_kernel(gsbn_que_args a) {
   __shared__ int sh_scan[1024 / 64 + 1];
+  const int hs = blockIdx.x;
   const int slot = a.exec % GSBN_HIST;
-  int32_t* out = a.hist_list + (int64_t)slot * a.Nsrc;
+  int32_t* out = a.hist_list + ((int64_t)slot * a.Hsrc + hs) * a.Msrc;
   int running = 0;
-  for (int base = 0; base < a.Nsrc; base += blockDim.x) {
-    const int n = base + threadIdx.x;
-    const int flag = (n < a.Nsrc && a.si[n] > 0) ? 1 : 0;
+  for (int base = 0; base < a.Msrc; base += blockDim.x) {
+    const int j = base + threadIdx.x;
+    const int flag = (j < a.Msrc && a.si[hs * a.Msrc + j] > 0) ? 1 : 0;
     int tot;
     const int pos = block_excl_scan(flag, sh_scan, &tot);
-    if (flag) out[running + pos] = n;
+    if (flag) out[running + pos] = j;
     running += tot;
   }
-  if (threadIdx.x == 0) a.hist_cnt[slot] = running;
+  if (threadIdx.x == 0) a.hist_cnt[(int64_t)slot * a.Hsrc + hs] = running;
 }
 
 __global__ void check_count_kernel(int n, const int8_t* spike, int32_t* count) {
@@ -552,6 +569,32 @@ __global__ void check_argmax_kernel(int H, int M, int32_t* count, int32_t* out_i
   }
   out_idx[h] = mi;
   out_cnt[h] = mc;
+}
+
+/* one workgroup per population */
+__global__ void check_count_multi_kernel(const gsbn_check_item* items) {
+  const gsbn_check_item it = items[blockIdx.x];
+  const int n = it.H * it.M;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) it.count[i] += (it.spike[i] > 0) ? 1 : 0;
+}
+
+/* one thread per HCU of one population (blockIdx.y) */
+__global__ void check_argmax_multi_kernel(const gsbn_check_item* items, int32_t* out_idx,
+                                          int32_t* out_cnt) {
+  const gsbn_check_item it = items[blockIdx.y];
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= it.H) return;
+  int mi = -1, mc = 0;
+  for (int j = 0; j < it.M; ++j) {
+    const int v = it.count[h * it.M + j];
+    if (v > mc) {
+      mi = j;
+      mc = v;
+    }
+    it.count[h * it.M + j] = 0;
+  }
+  out_idx[it.out_offset + h] = mi;
+  out_cnt[it.out_offset + h] = mc;
 }
 
 __global__ void syn_field_kernel(gsbn_syn_rec* syn, int64_t n, int field, uint32_t* buf,
@@ -675,17 +718,57 @@ int gsbn_proj_jxx(const gsbn_jxx_args* a, void* stream) {
 }
 
 int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream) {
-  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2) return GSBN_ERR_ARG;
-  const size_t lds = (size_t)2 * (a->H + 1) * sizeof(int);
+  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2 || a->parts < 0)
+    return GSBN_ERR_ARG;
+  gsbn_rowcol_args b = *a;
+  if (b.parts == 0) {
+    int p = 1536 / b.H;
+    b.parts = p < 1 ? 1 : (p > 32 ? 32 : p);
+  }
+  const int64_t grid = (int64_t)b.H * b.parts;
+  if (grid > 0x7fffffff) return GSBN_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(rowcol_kernel, dim3((unsigned)grid), dim3(RC_THREADS), 0,
+                     (hipStream_t)stream, b);
+  return launch_status();
+}
+
+int gsbn_pop_update_batch(const gsbn_pop_args* pops_dev, int32_t npop, const int32_t* wg_pop,
+                          const int32_t* wg_hcu, int32_t nwg, int32_t threads, int32_t max_slots,
+                          int32_t any_wta_pop_max, const gsbn_step_args* step, void* stream) {
+  if (!pops_dev || !wg_pop || !wg_hcu || !step || nwg <= 0 || npop <= 0 || threads <= 0 ||
+      threads > POP_MAX_THREADS || threads % 64 || max_slots < 0)
+    return GSBN_ERR_ARG;
+  const size_t lds = (size_t)((max_slots + 31) / 32) * sizeof(uint32_t);
   if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
-  const int grid = a->grid > 0 ? a->grid : 1024;
-  hipLaunchKernelGGL(rowcol_kernel, dim3(grid), dim3(RC_THREADS), lds, (hipStream_t)stream, *a);
+  hipStream_t s = (hipStream_t)stream;
+  if (any_wta_pop_max) {
+    hipLaunchKernelGGL(pop_dsup0_kernel_batch, dim3((npop + 63) / 64), dim3(64), 0, s, pops_dev,
+                       npop, *step);
+    int rc = launch_status();
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(pop_kernel_batch, dim3(nwg), dim3(threads), lds, s, pops_dev, wg_pop, wg_hcu,
+                     *step);
+  return launch_status();
+}
+
+int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
+                           const int32_t* wg_hcu, int32_t nhcu, int32_t parts,
+                           const gsbn_step_args* step, void* stream) {
+  if (!projs_dev || !wg_proj || !wg_hcu || !step || nhcu <= 0 || parts <= 0 ||
+      step->kp_slot < 0 || step->kp_slot >= GSBN_KP_SLOTS)
+    return GSBN_ERR_ARG;
+  const int64_t grid = (int64_t)nhcu * parts;
+  if (grid > 0x7fffffff) return GSBN_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(rowcol_kernel_batch, dim3((unsigned)grid), dim3(RC_THREADS), 0,
+                     (hipStream_t)stream, projs_dev, wg_proj, wg_hcu, parts, *step);
   return launch_status();
 }
 
 int gsbn_proj_que(const gsbn_que_args* a, void* stream) {
-  if (!a || a->Nsrc < 0 || a->exec < 0) return GSBN_ERR_ARG;
-  hipLaunchKernelGGL(que_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, *a);
+  if (!a || a->Hsrc <= 0 || a->Msrc <= 0 || a->exec < 0) return GSBN_ERR_ARG;
+  const int threads = a->Msrc >= 1024 ? 1024 : ((a->Msrc + 63) / 64) * 64;
+  hipLaunchKernelGGL(que_kernel, dim3(a->Hsrc), dim3(threads), 0, (hipStream_t)stream, *a);
   return launch_status();
 }
 
@@ -702,6 +785,22 @@ int gsbn_check_argmax(int32_t H, int32_t M, int32_t* count, int32_t* out_idx, in
   if (H <= 0 || M <= 0 || !count || !out_idx || !out_cnt) return GSBN_ERR_ARG;
   hipLaunchKernelGGL(check_argmax_kernel, dim3((H + 63) / 64), dim3(64), 0, (hipStream_t)stream,
                      H, M, count, out_idx, out_cnt);
+  return launch_status();
+}
+
+int gsbn_check_count_multi(const gsbn_check_item* items_dev, int32_t nitems, void* stream) {
+  if (!items_dev || nitems <= 0) return GSBN_ERR_ARG;
+  hipLaunchKernelGGL(check_count_multi_kernel, dim3(nitems), dim3(256), 0, (hipStream_t)stream,
+                     items_dev);
+  return launch_status();
+}
+
+int gsbn_check_argmax_multi(const gsbn_check_item* items_dev, int32_t nitems, int32_t* out_idx,
+                            int32_t* out_cnt, void* stream) {
+  if (!items_dev || nitems <= 0 || !out_idx || !out_cnt || nitems > 65535) return GSBN_ERR_ARG;
+  /* up to 64*16 = 1024 HCUs per population */
+  hipLaunchKernelGGL(check_argmax_multi_kernel, dim3(16, nitems), dim3(64), 0,
+                     (hipStream_t)stream, items_dev, out_idx, out_cnt);
   return launch_status();
 }
 

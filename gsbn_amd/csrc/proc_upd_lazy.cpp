@@ -19,6 +19,7 @@
  * closed-form catch-up uses the FP64-built coefficient table.
  */
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -70,8 +71,6 @@ class Pop {
  public:
   void init_new(ProcParam proc_param, PopParam pop_param, Database& db, vector<Pop*>* list_pop,
                 int* hcu_cnt, int* mcu_cnt);
-  void update(int simstep, int exec, float prn, int lginp_idx, int wmask_idx, bool do_jxx,
-              uint64_t seed, int wta_pop_max);
   void flush();
   gsbn_pop_args args(int simstep, int exec, float prn, int lginp_idx, int wmask_idx, bool do_jxx,
                      uint64_t seed, int wta_pop_max);
@@ -84,6 +83,8 @@ class Pop {
   SyncVector<int32_t>* _counter = nullptr;
   vector<Proj*> _in; /* incoming projections in proj_in_pop order */
   DevBuf<int32_t> spk_list, spk_cnt, ext_off, ext_units;
+  DevBuf<int32_t> hist_list, hist_cnt; /* spike-history ring, queue format */
+  bool write_hist = false;
   DevBuf<float> scratch;
   DevBuf<gsbn_pop_proj> projs_dev;
   DevBuf<int64_t> work;
@@ -99,6 +100,44 @@ struct CoefTable {
   bool saturated = false;
 };
 
+/* Coefficient tables shared by all projections with the same rates.  A
+ * table covers every gap up to the schedule's horizon or up to its
+ * saturation gap; `version` changes whenever a table is (re)built so cached
+ * launch arguments holding table pointers can be refreshed. */
+struct TableCache {
+  std::map<std::array<uint32_t, 4>, std::unique_ptr<CoefTable>> m;
+  uint64_t version = 0;
+  int sim_end = 0;
+  const CoefTable& get(float kzi, float kzj, float ke, float kp, int64_t need) {
+    std::array<uint32_t, 4> key;
+    memcpy(&key[0], &kzi, 4);
+    memcpy(&key[1], &kzj, 4);
+    memcpy(&key[2], &ke, 4);
+    memcpy(&key[3], &kp, 4);
+    auto& slot = m[key];
+    if (!slot) slot.reset(new CoefTable());
+    CoefTable& t = *slot;
+    if (t.saturated || t.len >= need) return t;
+    gsbn_rates_t r = {(double)kzi, (double)kzj, (double)ke, (double)kp};
+    const int64_t cap = (int64_t)1 << 26;
+    int64_t want = std::max<int64_t>(need, std::max<int64_t>(2 * t.len, (int64_t)sim_end + 2));
+    const int64_t sat = gsbn_coeff_saturation(&r, cap);
+    bool saturated = false;
+    if (sat >= 0 && sat + 1 <= want) {
+      want = sat + 1;
+      saturated = true;
+    }
+    if (want > cap) throw Error(-5, "coefficient table beyond 2^26 gaps");
+    vector<float> host((size_t)want * GSBN_COEF_STRIDE);
+    for (int64_t d = 0; d < want; ++d) gsbn_coeff_entry(&r, d, host.data() + d * GSBN_COEF_STRIDE);
+    t.dev.upload(host.data(), host.size());
+    t.len = want;
+    t.saturated = saturated;
+    ++version;
+    return t;
+  }
+};
+
 class Proj {
  public:
   void init_new(ProcParam proc_param, ProjParam proj_param, Database& db, vector<Proj*>* list_proj,
@@ -110,8 +149,12 @@ class Proj {
   void update_que(int exec);
   void mark_views_device();
   gsbn_pop_proj pop_desc();
-  const CoefTable& table(float kp, int64_t need);
+  const CoefTable& table(float kp, int64_t need) {
+    return tcache->get(_tauzidt, _tauzjdt, _tauedt, kp, need);
+  }
+  gsbn_rowcol_args static_args(const float* slot_prn, int nslots, int64_t need);
   void rebuild_fanout();
+  TableCache* tcache = nullptr;
 
   int _id = 0, _rank = 0, _proj_in_pop = 0, _dim_conn = 0, _dim_hcu = 0, _dim_mcu = 0, _nsrc = 0;
   float _taupdt = 0, _tauedt = 0, _tauzidt = 0, _tauzjdt = 0, _tauepscdt = 0, _eps = 0, _eps2 = 0;
@@ -130,10 +173,12 @@ class Proj {
   DevBuf<gsbn_row_rec> rowst;
   DevBuf<float> contrib;
   DevBuf<int32_t> rows_list, rows_cnt, pend_cnt, act_stamp, hist_list, hist_cnt, inv_off, inv_slot;
-  std::map<uint32_t, std::unique_ptr<CoefTable>> tables;
   uint64_t ii_version = ~0ull, di_version = ~0ull;
   int dmin = 1, dmax = 0;
   int last_exec = -1;
+  bool direct_que = false; /* queue pushes read from the source population's history */
+  const int32_t* ring_list() const { return direct_que ? _ptr_src_pop->hist_list.p : hist_list.p; }
+  const int32_t* ring_cnt() const { return direct_que ? _ptr_src_pop->hist_cnt.p : hist_cnt.p; }
   int sim_end = 0;
   std::vector<std::unique_ptr<VecView>> view_objs;
 };
@@ -238,21 +283,23 @@ void QueueView::pull(VecBase* v) {
   /* qi / siq from the ring: que at exec e pushed bit (di-1); after the que
    * of exec x it sits at bit (di-1)-(x-e); siq(x) = pushed at x-di */
   const int x = p->last_exec;
-  vector<int32_t> hc(GSBN_HIST);
-  p->hist_cnt.download(hc.data(), GSBN_HIST);
+  const int Hs = p->_ptr_src_pop->_dim_hcu, Ms = p->_ptr_src_pop->_dim_mcu;
+  vector<int32_t> hc((size_t)GSBN_HIST * Hs), hl((size_t)GSBN_HIST * Hs * Ms);
+  hip_check(hipStreamSynchronize(active_stream()), "sync");
+  hip_check(hipMemcpy(hc.data(), p->ring_cnt(), hc.size() * 4, hipMemcpyDeviceToHost), "D2H");
+  hip_check(hipMemcpy(hl.data(), p->ring_list(), hl.size() * 4, hipMemcpyDeviceToHost), "D2H");
   const int lo = std::max(0, x - 32);
   const int span = x - lo + 1;
   vector<uint8_t> present;
   if (x >= 0) {
     present.assign((size_t)span * p->_nsrc, 0);
-    vector<int32_t> row(p->_nsrc);
     for (int e = lo; e <= x; ++e) {
       const int slot = e % GSBN_HIST;
-      if (hc[slot] <= 0) continue;
-      hip_check(hipStreamSynchronize(active_stream()), "sync");
-      hip_check(hipMemcpy(row.data(), p->hist_list.p + (int64_t)slot * p->_nsrc, hc[slot] * 4,
-                          hipMemcpyDeviceToHost), "D2H");
-      for (int i = 0; i < hc[slot]; ++i) present[(size_t)(e - lo) * p->_nsrc + row[i]] = 1;
+      for (int hs = 0; hs < Hs; ++hs) {
+        const int c = hc[(size_t)slot * Hs + hs];
+        for (int i = 0; i < c; ++i)
+          present[(size_t)(e - lo) * p->_nsrc + hs * Ms + hl[((size_t)slot * Hs + hs) * Ms + i]] = 1;
+      }
     }
   }
   const int* ii = p->_ii->cpu_data();
@@ -337,6 +384,9 @@ void Pop::init_new(ProcParam proc_param, PopParam pop_param, Database& db, vecto
   spk_list.alloc(N);
   spk_cnt.alloc(_dim_hcu);
   op_check(gsbn_i32_fill(spk_cnt.p, _dim_hcu, 0, active_stream()), "fill");
+  hist_list.alloc((size_t)GSBN_HIST * N);
+  hist_cnt.alloc((size_t)GSBN_HIST * _dim_hcu);
+  op_check(gsbn_i32_fill(hist_cnt.p, (int64_t)GSBN_HIST * _dim_hcu, 0, active_stream()), "fill");
   scratch.alloc(1);
   /* external spikes for debugging (Pop.cpp:67-110): lines step,pop,idx,... */
   Parser par(proc_param);
@@ -422,6 +472,10 @@ gsbn_pop_args Pop::args(int simstep, int exec, float prn, int lginp_idx, int wma
   }
   a.spk_list = spk_list.p;
   a.spk_cnt = spk_cnt.p;
+  if (write_hist) {
+    a.hist_out_list = hist_list.p;
+    a.hist_out_cnt = hist_cnt.p;
+  }
   a.nproj_in = (int)_in.size();
   a.do_jxx = do_jxx ? 1 : 0;
   int max_slots = 0;
@@ -444,15 +498,6 @@ gsbn_pop_args Pop::args(int simstep, int exec, float prn, int lginp_idx, int wma
   }
   a.work = work.p;
   return a;
-}
-
-void Pop::update(int simstep, int exec, float prn, int lginp_idx, int wmask_idx, bool do_jxx,
-                 uint64_t seed, int wta_pop_max) {
-  gsbn_pop_args a = args(simstep, exec, prn, lginp_idx, wmask_idx, do_jxx, seed, wta_pop_max);
-  Timing::get().begin(0, active_stream());
-  op_check(gsbn_pop_update(&a, active_stream()), "gsbn_pop_update");
-  Timing::get().end(0, active_stream());
-  flush_pending = true;
 }
 
 /* apply pending support contributions before a host read of epsc */
@@ -565,11 +610,12 @@ void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
   pend_cnt.alloc(_dim_hcu);
   act_stamp.alloc((size_t)R);
   hist_list.alloc((size_t)GSBN_HIST * std::max(_nsrc, 1));
-  hist_cnt.alloc(GSBN_HIST);
+  hist_cnt.alloc((size_t)GSBN_HIST * _ptr_src_pop->_dim_hcu);
   op_check(gsbn_i32_fill(rows_cnt.p, _dim_hcu, 0, active_stream()), "fill");
   op_check(gsbn_i32_fill(pend_cnt.p, _dim_hcu, 0, active_stream()), "fill");
   op_check(gsbn_i32_fill(act_stamp.p, R, -1, active_stream()), "fill");
-  op_check(gsbn_i32_fill(hist_cnt.p, GSBN_HIST, 0, active_stream()), "fill");
+  op_check(gsbn_i32_fill(hist_cnt.p, (int64_t)GSBN_HIST * _ptr_src_pop->_dim_hcu, 0,
+                         active_stream()), "fill");
 
   /* views over the device state */
   struct VB {
@@ -615,8 +661,10 @@ gsbn_pop_proj Proj::pop_desc() {
   d.kftj = _kftj;
   d.bgain = _bgain;
   d.eps = _eps;
-  d.hist_list = hist_list.p;
-  d.hist_cnt = hist_cnt.p;
+  d.hist_list = ring_list();
+  d.hist_cnt = ring_cnt();
+  d.Hsrc = _ptr_src_pop->_dim_hcu;
+  d.Msrc = _ptr_src_pop->_dim_mcu;
   d.Nsrc = _nsrc;
   d.dmin = dmin;
   d.dmax = dmax;
@@ -670,31 +718,6 @@ void Proj::rebuild_fanout() {
   inv_off.upload(off.data(), off.size());
   if (slots.empty()) slots.push_back(0);
   inv_slot.upload(slots.data(), slots.size());
-}
-
-const CoefTable& Proj::table(float kp, int64_t need) {
-  uint32_t key;
-  memcpy(&key, &kp, 4);
-  auto& slot = tables[key];
-  if (!slot) slot.reset(new CoefTable());
-  CoefTable& t = *slot;
-  if (t.saturated || t.len >= need) return t;
-  gsbn_rates_t r = {(double)_tauzidt, (double)_tauzjdt, (double)_tauedt, (double)kp};
-  const int64_t cap = (int64_t)1 << 26;
-  int64_t want = std::max<int64_t>(need, std::max<int64_t>(2 * t.len, (int64_t)sim_end + 2));
-  const int64_t sat = gsbn_coeff_saturation(&r, cap);
-  bool saturated = false;
-  if (sat >= 0 && sat + 1 <= want) {
-    want = sat + 1;
-    saturated = true;
-  }
-  if (want > cap) throw Error(-5, "coefficient table beyond 2^26 gaps");
-  vector<float> host((size_t)want * GSBN_COEF_STRIDE);
-  for (int64_t d = 0; d < want; ++d) gsbn_coeff_entry(&r, d, host.data() + d * GSBN_COEF_STRIDE);
-  t.dev.upload(host.data(), host.size());
-  t.len = want;
-  t.saturated = saturated;
-  return t;
 }
 
 void Proj::prepare(int simstep) {
@@ -762,25 +785,20 @@ void Proj::update_jxx(float prn) {
   Timing::get().end(3, active_stream());
 }
 
-void Proj::update_rowcol(int simstep, int exec, float prn) {
+/* launch arguments that stay fixed between refreshes; the batched and the
+ * single launch fill simstep/exec/kp/table per step */
+gsbn_rowcol_args Proj::static_args(const float* slot_prn, int nslots, int64_t need) {
   Pop* dp = _ptr_dest_pop;
-  const float kp = _taupdt * prn;
-  const CoefTable& t = table(kp, (int64_t)simstep + 2);
   gsbn_rowcol_args a;
   memset(&a, 0, sizeof(a));
   a.H = _dim_hcu;
   a.M = _dim_mcu;
   a.C = _dim_conn;
-  a.simstep = simstep;
-  a.exec = exec;
-  a.kp = kp;
   a.kfti = _kfti;
   a.kftj = _kftj;
   a.wgain = _wgain;
   a.eps = _eps;
   a.eps2 = _eps2;
-  a.table = t.dev.p;
-  a.table_len = (int32_t)t.len;
   a.syn = syn.p;
   a.rowst = rowst.p;
   a.pj = _pj->gpu_data();
@@ -792,14 +810,21 @@ void Proj::update_rowcol(int simstep, int exec, float prn) {
   a.spk_cnt = dp->spk_cnt.p;
   a.act_stamp = act_stamp.p;
   a.contrib = contrib.p;
-  Timing::get().begin(1, active_stream());
-  op_check(gsbn_proj_rowcol(&a, active_stream()), "gsbn_proj_rowcol");
-  Timing::get().end(1, active_stream());
+  a.taupdt = _taupdt;
+  for (int i = 0; i < nslots; ++i) {
+    const CoefTable& t = table(_taupdt * slot_prn[i], need);
+    a.tables[i] = t.dev.p;
+    a.table_lens[i] = (int32_t)t.len;
+  }
+  return a;
 }
 
 void Proj::update_que(int exec) {
+  last_exec = exec;
+  if (direct_que) return; /* the population kernel already stored si of the next step */
   gsbn_que_args a;
-  a.Nsrc = _nsrc;
+  a.Hsrc = _ptr_src_pop->_dim_hcu;
+  a.Msrc = _ptr_src_pop->_dim_mcu;
   a.exec = exec;
   a.si = _si->gpu_data();
   a.hist_list = hist_list.p;
@@ -807,7 +832,6 @@ void Proj::update_que(int exec) {
   Timing::get().begin(4, active_stream());
   op_check(gsbn_proj_que(&a, active_stream()), "gsbn_proj_que");
   Timing::get().end(4, active_stream());
-  last_exec = exec;
 }
 
 void Proj::mark_views_device() {
@@ -828,12 +852,31 @@ class ProcUpdLazy : public ProcedureBase {
   int64_t sweep_synapses = 0, sweeps = 0, execs = 0;
 
  private:
+  int kp_slot(float prn);
+  void refresh_args(int simstep);
+  void build_batch_maps();
   vector<Proj*> _list_proj;
   vector<Pop*> _list_pop;
   GlobalVar _glv;
   uint64_t _seed = 0;
   int _wta_pop_max = 0;
   int _exec = 0;
+  bool _configured = false;
+  TableCache _tables;
+  /* cached launch arguments */
+  bool _batch = false;
+  bool _args_dirty = true;
+  uint64_t _epoch_seen = ~0ull, _tables_seen = ~0ull;
+  vector<gsbn_pop_args> _pop_args;
+  vector<gsbn_rowcol_args> _proj_args;
+  float _slot_prn[GSBN_KP_SLOTS] = {0, 0, 0, 0};
+  int _nslots = 0, _next_evict = 0;
+  int _total_units = 0, _total_hcus = 0;
+  /* batched launch state */
+  DevBuf<gsbn_pop_args> _pop_dev;
+  DevBuf<gsbn_rowcol_args> _proj_dev;
+  DevBuf<int32_t> _wg_pop, _wg_pop_hcu, _wg_proj, _wg_proj_hcu;
+  int _pop_threads = 64, _max_slots = 0, _any_quirk = 0, _nwg_pop = 0, _nhcu_proj = 0, _parts = 1;
 };
 REGISTERIMPL(ProcUpdLazy);
 
@@ -864,9 +907,13 @@ void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
     }
   }
   const int total_pop_num = (int)_list_pop.size();
+  _total_units = mcu_cnt;
+  _total_hcus = hcu_cnt;
+  if (!_glv.geti("sim-end-step", _tables.sim_end)) _tables.sim_end = 0;
   for (auto& jp : net_param.proj_param) {
     if ((int)jp.src_pop < total_pop_num && (int)jp.dest_pop < total_pop_num) {
       Proj* proj = new Proj();
+      proj->tcache = &_tables;
       try {
         proj->init_new(proc_param, jp, db, &_list_proj, &_list_pop);
       } catch (...) {
@@ -876,6 +923,7 @@ void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
     }
   }
   hip_check(hipStreamSynchronize(active_stream()), "sync");
+  _glv.puti("updlazy-ready", 1);
 }
 
 void ProcUpdLazy::init_copy(SolverParam, Database&) {
@@ -884,6 +932,64 @@ void ProcUpdLazy::init_copy(SolverParam, Database&) {
 
 void ProcUpdLazy::update_cpu() {
   throw Error(-5, "ProcUpdLazy::update_cpu: this build runs the lazy update on the GPU only");
+}
+
+int ProcUpdLazy::kp_slot(float prn) {
+  for (int i = 0; i < _nslots; ++i)
+    if (!memcmp(&_slot_prn[i], &prn, 4)) return i;
+  _args_dirty = true;
+  if (_nslots < GSBN_KP_SLOTS) {
+    _slot_prn[_nslots] = prn;
+    return _nslots++;
+  }
+  const int i = _next_evict;
+  _next_evict = (i + 1) % GSBN_KP_SLOTS;
+  _slot_prn[i] = prn;
+  return i;
+}
+
+void ProcUpdLazy::build_batch_maps() {
+  vector<int32_t> wp, wh, qp, qh;
+  _pop_threads = 64;
+  _any_quirk = _wta_pop_max;
+  for (size_t i = 0; i < _list_pop.size(); ++i) {
+    Pop* p = _list_pop[i];
+    for (int h = 0; h < p->_dim_hcu; ++h) {
+      wp.push_back((int32_t)i);
+      wh.push_back(h);
+    }
+    _pop_threads = std::max(_pop_threads, ((p->_dim_mcu + 63) / 64) * 64);
+  }
+  _max_slots = 0;
+  for (size_t i = 0; i < _list_proj.size(); ++i) {
+    Proj* q = _list_proj[i];
+    for (int h = 0; h < q->_dim_hcu; ++h) {
+      qp.push_back((int32_t)i);
+      qh.push_back(h);
+    }
+    _max_slots = std::max(_max_slots, q->_dim_conn);
+  }
+  _nwg_pop = (int)wp.size();
+  _nhcu_proj = (int)qp.size();
+  _parts = std::max(1, std::min(32, 1536 / std::max(_nhcu_proj, 1)));
+  _wg_pop.upload(wp.data(), wp.size());
+  _wg_pop_hcu.upload(wh.data(), wh.size());
+  if (!qp.empty()) {
+    _wg_proj.upload(qp.data(), qp.size());
+    _wg_proj_hcu.upload(qh.data(), qh.size());
+  }
+}
+
+void ProcUpdLazy::refresh_args(int simstep) {
+  _pop_args.clear();
+  for (auto* p : _list_pop) _pop_args.push_back(p->args(0, 0, 0.0f, 0, 0, true, _seed, _wta_pop_max));
+  _proj_args.clear();
+  for (auto* q : _list_proj) _proj_args.push_back(q->static_args(_slot_prn, _nslots, (int64_t)simstep + 2));
+  if (_batch) {
+    _pop_dev.upload(_pop_args.data(), _pop_args.size());
+    if (!_proj_args.empty()) _proj_dev.upload(_proj_args.data(), _proj_args.size());
+  }
+  _args_dirty = false;
 }
 
 void ProcUpdLazy::update_gpu() {
@@ -897,10 +1003,67 @@ void ProcUpdLazy::update_gpu() {
   GSBN_CHECK(_glv.getf("old-prn", old_prn), "old-prn");
   GSBN_CHECK(_glv.geti("lginp-idx", lginp_idx), "lginp-idx");
   GSBN_CHECK(_glv.geti("wmask-idx", wmask_idx), "wmask-idx");
-  for (auto* q : _list_proj) q->prepare(simstep);
+  hipStream_t stream = active_stream();
+  if (!_configured) {
+    /* ProcExchangeSpike initialised after this procedure delivers
+     * spike_<src> to si_<proj> once per executed step, so the queue push
+     * of step x sees the spikes of step x-1: the population kernel stores
+     * them in its history ring directly */
+    int direct = 0;
+    _glv.geti("exchange-spike-direct", direct);
+    for (auto* q : _list_proj) {
+      q->direct_que = direct != 0;
+      if (q->direct_que) q->_ptr_src_pop->write_hist = true;
+    }
+    _batch = _list_pop.size() > 1 || _list_proj.size() > 1;
+    if (_batch) build_batch_maps();
+    _configured = true;
+    _args_dirty = true;
+  }
+  const int slot = kp_slot(prn);
+  for (auto* q : _list_proj) (void)q->table(q->_taupdt * prn, (int64_t)simstep + 2);
+  const bool touched = VecBase::host_epoch() != _epoch_seen;
+  if (touched)
+    for (auto* q : _list_proj) q->prepare(simstep);
+  if (touched || _args_dirty || _tables.version != _tables_seen) refresh_args(simstep);
+
+  /* current .lginp / .wmask rows (Pop.cpp:331-332) */
+  Pop* p0 = _list_pop.front();
+  const int64_t lg_off = p0->_lginp->ld() <= 0 ? (int64_t)lginp_idx : (int64_t)lginp_idx * p0->_lginp->ld();
+  const int64_t wm_off = p0->_wmask->ld() <= 0 ? (int64_t)wmask_idx : (int64_t)wmask_idx * p0->_wmask->ld();
+  if (lginp_idx < 0 || lg_off + _total_units > (int64_t)p0->_lginp->size())
+    throw Error(-2, "lginp-idx out of range");
+  if (wmask_idx < 0 || wm_off + _total_hcus > (int64_t)p0->_wmask->size())
+    throw Error(-2, "wmask-idx " + std::to_string(wmask_idx) + " out of range");
+
   const bool sweep = old_prn != prn;
-  for (auto* p : _list_pop)
-    p->update(simstep, _exec, prn, lginp_idx, wmask_idx, !sweep, _seed, _wta_pop_max);
+  gsbn_step_args st;
+  memset(&st, 0, sizeof(st));
+  st.simstep = simstep;
+  st.exec = _exec;
+  st.do_jxx = sweep ? 0 : 1;
+  st.kp_slot = slot;
+  st.prn = prn;
+  st.lginp_offset = lg_off;
+  st.wmask_offset = wm_off;
+
+  Timing::get().begin(0, stream);
+  if (_batch) {
+    op_check(gsbn_pop_update_batch(_pop_dev.p, (int32_t)_pop_args.size(), _wg_pop.p, _wg_pop_hcu.p,
+                                   _nwg_pop, _pop_threads, _max_slots, _any_quirk, &st, stream),
+             "gsbn_pop_update_batch");
+  } else {
+    gsbn_pop_args a = _pop_args[0];
+    a.simstep = simstep;
+    a.exec = _exec;
+    a.prn = prn;
+    a.do_jxx = st.do_jxx;
+    a.lginp += lg_off;
+    a.wmask += wm_off;
+    op_check(gsbn_pop_update(&a, stream), "gsbn_pop_update");
+  }
+  Timing::get().end(0, stream);
+  for (auto* p : _list_pop) p->flush_pending = true;
   if (sweep) {
     for (auto* q : _list_proj) {
       q->update_all(simstep, old_prn);
@@ -909,9 +1072,28 @@ void ProcUpdLazy::update_gpu() {
     sweeps++;
     for (auto* q : _list_proj) q->update_jxx(prn);
   }
-  for (auto* q : _list_proj) q->update_rowcol(simstep, _exec, prn);
+  if (!_list_proj.empty()) {
+    Timing::get().begin(1, stream);
+    if (_batch) {
+      op_check(gsbn_proj_rowcol_batch(_proj_dev.p, _wg_proj.p, _wg_proj_hcu.p, _nhcu_proj, _parts,
+                                      &st, stream),
+               "gsbn_proj_rowcol_batch");
+    } else {
+      gsbn_rowcol_args a = _proj_args[0];
+      a.simstep = simstep;
+      a.exec = _exec;
+      a.kp = a.taupdt * prn;
+      a.table = a.tables[slot];
+      a.table_len = a.table_lens[slot];
+      op_check(gsbn_proj_rowcol(&a, stream), "gsbn_proj_rowcol");
+    }
+    Timing::get().end(1, stream);
+  }
   for (auto* q : _list_proj) q->update_que(_exec);
-  for (auto* q : _list_proj) q->mark_views_device();
+  if (touched)
+    for (auto* q : _list_proj) q->mark_views_device();
+  _epoch_seen = VecBase::host_epoch();
+  _tables_seen = _tables.version;
   _exec++;
   execs++;
 }

@@ -249,6 +249,7 @@ class ProcExchangeSpike : public ProcedureBase {
     SyncVector<int8_t>* si;
   };
   vector<Link> _links;
+  bool _direct = false;
 };
 REGISTERIMPL(ProcExchangeSpike);
 
@@ -270,12 +271,37 @@ void ProcExchangeSpike::init_new(SolverParam solver_param, Database& db) {
     _links.push_back(l);
     ++id;
   }
+  /* initialised after ProcUpdLazy: every executed step delivers the spikes
+   * of that step to si after the lazy update, so the next step's queue push
+   * sees them; ProcUpdLazy then takes the push from its own spike history */
+  int ready = 0;
+  if (_glv.geti("updlazy-ready", ready) && ready) {
+    _glv.puti("exchange-spike-direct", 1);
+    _direct = true;
+    /* si_<proj> is then only materialised when the host reads it: it equals
+     * the source population's spike vector of the last executed step */
+    for (auto& l : _links) {
+      (void)l.si->mutable_gpu_data();
+      SyncVector<int8_t>* si = l.si;
+      SyncVector<int8_t>* sj = l.sj;
+      si->before_host_read = [si, sj]() {
+        if (si->size())
+          hip_check(hipMemcpyAsync(si->mutable_gpu_data(), sj->gpu_data(), si->size(),
+                                   hipMemcpyDeviceToDevice, active_stream()),
+                    "exchange copy");
+      };
+    }
+  }
 }
 
 void ProcExchangeSpike::update_gpu() {
   int cycle_flag = 0;
   GSBN_CHECK(_glv.geti("cycle-flag", cycle_flag), "cycle-flag");
   if (cycle_flag != 1) return;
+  if (_direct) {
+    for (auto& l : _links) l.si->mark_device();
+    return;
+  }
   for (auto& l : _links) {
     const int8_t* src = l.sj->gpu_data();
     int8_t* dst = l.si->mutable_gpu_data();
@@ -322,6 +348,8 @@ class ProcCheck : public ProcedureBase {
   int _reported = 0;
   int _correct = 0, _total = 0;
   vector<int> _flags;
+  vector<gsbn_check_item> _items;
+  gsbn_check_item* _items_dev = nullptr;
   bool _final_done = false;
 };
 REGISTERIMPL(ProcCheck);
@@ -330,6 +358,7 @@ ProcCheck::~ProcCheck() {
   for (auto& p : _pops)
     if (p.count) (void)hipFree(p.count);
   if (_res_idx) (void)hipFree(_res_idx);
+  if (_items_dev) (void)hipFree(_items_dev);
   if (_res_cnt) (void)hipFree(_res_cnt);
 }
 
@@ -402,9 +431,35 @@ void ProcCheck::update_gpu() {
   const mode_t& m = _list_mode[_cursor];
   if (timestep <= m.begin_step) return;
   if (timestep < m.end_step) {
+    bool multi = _pops.size() > 1;
     for (auto& p : _pops)
-      op_check(gsbn_check_count(p.H * p.M, p.spike->gpu_data(), p.count, active_stream()),
-               "gsbn_check_count");
+      if (p.H > 1024) multi = false; /* gsbn_check_argmax_multi covers 1024 HCUs */
+    if (!multi) {
+      for (auto& p : _pops)
+        op_check(gsbn_check_count(p.H * p.M, p.spike->gpu_data(), p.count, active_stream()),
+                 "gsbn_check_count");
+    } else {
+      if (_items.empty()) {
+        vector<gsbn_check_item> it;
+        int hoff = 0;
+        for (auto& p : _pops) {
+          gsbn_check_item c;
+          c.spike = p.spike->gpu_data();
+          c.count = p.count;
+          c.H = p.H;
+          c.M = p.M;
+          c.out_offset = hoff;
+          hoff += p.H;
+          it.push_back(c);
+        }
+        _items = it;
+        hip_check(hipMalloc(&_items_dev, sizeof(gsbn_check_item) * it.size()), "hipMalloc");
+        hip_check(hipMemcpy(_items_dev, it.data(), sizeof(gsbn_check_item) * it.size(),
+                            hipMemcpyHostToDevice), "H2D");
+      }
+      op_check(gsbn_check_count_multi(_items_dev, (int32_t)_items.size(), active_stream()),
+               "gsbn_check_count_multi");
+    }
     _updated_flag = true;
     return;
   }
@@ -413,12 +468,19 @@ void ProcCheck::update_gpu() {
     return;
   }
   const int w = (int)_window_mode.size();
-  int hoff = 0;
-  for (auto& p : _pops) {
-    op_check(gsbn_check_argmax(p.H, p.M, p.count, _res_idx + (int64_t)w * _total_hcu + hoff,
-                               _res_cnt + (int64_t)w * _total_hcu + hoff, active_stream()),
-             "gsbn_check_argmax");
-    hoff += p.H;
+  if (_items_dev) {
+    op_check(gsbn_check_argmax_multi(_items_dev, (int32_t)_items.size(),
+                                     _res_idx + (int64_t)w * _total_hcu,
+                                     _res_cnt + (int64_t)w * _total_hcu, active_stream()),
+             "gsbn_check_argmax_multi");
+  } else {
+    int hoff = 0;
+    for (auto& p : _pops) {
+      op_check(gsbn_check_argmax(p.H, p.M, p.count, _res_idx + (int64_t)w * _total_hcu + hoff,
+                                 _res_cnt + (int64_t)w * _total_hcu + hoff, active_stream()),
+               "gsbn_check_argmax");
+      hoff += p.H;
+    }
   }
   _window_mode.push_back(_cursor);
   _cursor++;

@@ -53,6 +53,17 @@ typedef struct {
 
 #define GSBN_HIST 64 /* exec steps kept in the spike queue ring (delays <= 31) */
 #define GSBN_MAX_PROJ_IN 8
+#define GSBN_KP_SLOTS 4 /* distinct learning rates a batched projection can switch between */
+
+/* per-step fields shared by all populations / projections of a batch that
+ * run in lock-step (one schedule) */
+typedef struct {
+  int32_t simstep, exec, do_jxx, kp_slot;
+  float prn;
+  int32_t pad;
+  int64_t lginp_offset; /* added to every population's lginp pointer */
+  int64_t wmask_offset; /* added to every population's wmask pointer */
+} gsbn_step_args;
 
 /* one projection entering a population, as seen by gsbn_pop_update */
 typedef struct {
@@ -66,11 +77,12 @@ typedef struct {
   float* ej;
   float* zj;
   float taupdt, ke, kzj, kepsc, kftj, bgain, eps;
-  /* spike queue: hist_list[(e % GSBN_HIST)*Nsrc + i], i < hist_cnt[e % GSBN_HIST],
-   * source units whose si was set at the queue update of exec step e */
+  /* spike queue ring, per source HCU: at the queue update of exec step e
+   * (slot = e % GSBN_HIST) source HCU hs had si set for the MCUs
+   * hist_list[(slot*Hsrc + hs)*Msrc + i], i < hist_cnt[slot*Hsrc + hs] */
   const int32_t* hist_list;
   const int32_t* hist_cnt;
-  int32_t Nsrc, dmin, dmax;
+  int32_t Hsrc, Msrc, Nsrc, dmin, dmax;
   /* inverse connectivity: for destination HCU h and source unit n the slots
    * inv_slot[inv_off[h*(Nsrc+1)+n] .. inv_off[h*(Nsrc+1)+n+1]) hold ii == n */
   const int32_t* inv_off;
@@ -111,6 +123,12 @@ typedef struct {
   int32_t ext_nsteps;
   int32_t* spk_list; /* [H][M] spiking units of each HCU, ascending */
   int32_t* spk_cnt;  /* [H] */
+  /* optional spike-history ring of this population in the queue format: the
+   * spikes of exec step x are stored at slot (x+1) % GSBN_HIST, which is what
+   * the queue update of step x+1 sees when ProcExchangeSpike delivers
+   * spike_<pop> to si_<proj> once per step; NULL to skip */
+  int32_t* hist_out_list;
+  int32_t* hist_out_cnt;
   int32_t nproj_in, do_jxx;
   int32_t max_slots; /* largest C among the incoming projections (sizes the row bitmap) */
   const gsbn_pop_proj* projs; /* device array, index = position of the projection in the population */
@@ -145,7 +163,10 @@ typedef struct {
 } gsbn_jxx_args;
 int gsbn_proj_jxx(const gsbn_jxx_args* a, void* stream);
 
-/* Row and column lazy updates of one step (Proj.cpp:365-536, 667-760):
+/* Row and column lazy updates of one step (Proj.cpp:365-536, 667-760).
+ * Grid: `parts` workgroups per destination HCU; part q of HCU h takes the
+ * active rows q, q+parts, ... of h and the slice q of the C slots of every
+ * spiking unit of h.
  * active rows catch up with kfti on zi/zi2, Wij when kp != 0, and write their
  * support contribution; columns of spiking postsynaptic units catch up with
  * kftj on zj2 (slots with ii < 0 skipped).  A synapse in both an active row
@@ -167,14 +188,29 @@ typedef struct {
   const int32_t* spk_cnt;
   const int32_t* act_stamp;
   float* contrib;
-  int32_t grid; /* workgroups to launch (0: default) */
+  int32_t parts; /* workgroups per HCU (0: chosen from H) */
+  /* batched form: kp = taupdt * step.prn and the table of step.kp_slot */
+  float taupdt;
+  const float* tables[GSBN_KP_SLOTS];
+  int32_t table_lens[GSBN_KP_SLOTS];
 } gsbn_rowcol_args;
 int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream);
 
+/* Batched forms: many populations / projections (e.g. independent networks)
+ * in one launch.  The argument structs live in device memory (their per-step
+ * fields are taken from `step`); wg_pop/wg_hcu (wg_proj/wg_hcu) map each
+ * workgroup (HCU) to its population (projection) and local HCU. */
+int gsbn_pop_update_batch(const gsbn_pop_args* pops_dev, int32_t npop, const int32_t* wg_pop,
+                          const int32_t* wg_hcu, int32_t nwg, int32_t threads, int32_t max_slots,
+                          int32_t any_wta_pop_max, const gsbn_step_args* step, void* stream);
+int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
+                           const int32_t* wg_hcu, int32_t nhcu, int32_t parts,
+                           const gsbn_step_args* step, void* stream);
+
 /* Delay-queue push (Proj.cpp:645-664): record the source units with si > 0
- * for exec step `exec`. */
+ * for exec step `exec`, per source HCU.  Grid: one workgroup per source HCU. */
 typedef struct {
-  int32_t Nsrc, exec;
+  int32_t Hsrc, Msrc, exec;
   const int8_t* si;
   int32_t* hist_list;
   int32_t* hist_cnt;
@@ -186,6 +222,16 @@ int gsbn_proj_que(const gsbn_que_args* a, void* stream);
 int gsbn_check_count(int32_t n, const int8_t* spike, int32_t* count, void* stream);
 int gsbn_check_argmax(int32_t H, int32_t M, int32_t* count, int32_t* out_idx, int32_t* out_cnt,
                       void* stream);
+/* the same over several populations in one launch; item i writes its H
+ * results at out_idx/out_cnt + out_offset */
+typedef struct {
+  const int8_t* spike;
+  int32_t* count;
+  int32_t H, M, out_offset, pad;
+} gsbn_check_item;
+int gsbn_check_count_multi(const gsbn_check_item* items_dev, int32_t nitems, void* stream);
+int gsbn_check_argmax_multi(const gsbn_check_item* items_dev, int32_t nitems, int32_t* out_idx,
+                            int32_t* out_cnt, void* stream);
 
 /* field gather/scatter between the record arrays and plain vectors
  * field: 0 pij 1 eij 2 zi2 3 zj2 4 wij 5 tij (syn); 0 pi 1 ei 2 zi 3 ti (row) */

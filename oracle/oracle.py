@@ -312,6 +312,11 @@ class OracleNet:
     def _init_conn(self):
         """ProcInitConn::add_row (ProcInitConn.cpp:92-103): fill the first free
         slot of the destination HCU, in file order."""
+        if self.spec.conn_arrays is not None:
+            for pid, (ii, di) in enumerate(self.spec.conn_arrays):
+                self.write("ii_%d" % pid, ii)
+                self.write("di_%d" % pid, di)
+            return
         for pid, rows in enumerate(self.spec.conn):
             ii = self.vector("ii_%d" % pid)
             di = self.vector("di_%d" % pid)

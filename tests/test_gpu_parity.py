@@ -51,3 +51,59 @@ def test_config0_full_run_bitwise(tmp_path):
     assert (c, t) == (ora.check.correct, ora.check.total)
     assert list(flags) == [int(r["ok"]) for r in ora.check.results]
     gpu.close()
+
+
+def _lockstep_and_compare(spec, tmp_path, max_steps=None):
+    gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path, max_steps=max_steps)
+    assert div is None, "spike rasters diverge at step %s" % (div,)
+    bad = helpers.compare_state(gpu, ora, spec)
+    assert not bad, bad
+    return gpu, ora
+
+
+def test_config0_queue_kernel_path(tmp_path):
+    # ProcExchangeSpike listed before ProcUpdLazy: the queue push runs as its
+    # own kernel from si_<proj>; same results as the direct path
+    spec = configs.config0()
+    spec.exchange_first = True
+    gpu, ora = _lockstep_and_compare(spec, tmp_path, max_steps=600)
+    gpu.close()
+
+
+def test_config0_wta_population_maximum(tmp_path):
+    spec = configs.config0()
+    spec.wta_pop_max = 1
+    gpu, ora = _lockstep_and_compare(spec, tmp_path, max_steps=700)
+    gpu.close()
+
+
+def test_external_spikes(tmp_path):
+    rng = np.random.default_rng(5)
+    spec = configs.config0()
+    spec.ext_spikes = {0: {s: sorted(rng.choice(100, size=rng.integers(0, 6), replace=False).tolist())
+                           for s in range(1, 400)}}
+    gpu, ora = _lockstep_and_compare(spec, tmp_path, max_steps=400)
+    gpu.close()
+
+
+def test_reference_sparse_delay_map(tmp_path):
+    # 400 HCU x 10 MCU with the reference's data/init_conn_0.1.map: up to 60
+    # slots per HCU, delays 1..4, many empty slots (ii < 0)
+    import os
+    text = open(os.path.join(os.path.dirname(__file__), "golden", "init_conn_0.1.map")).read()
+    spec = configs.recall_net(400, 10, 4, 60, 10, 60, 0.5, reset=20, name="sparse")
+    spec.projs[0].slot_num = 60.0
+    spec.conn_text = text
+    spec.conn = configs.parse_conn_map(text)
+    gpu, ora = _lockstep_and_compare(spec, tmp_path, max_steps=450)
+    gpu.close()
+
+
+def test_config1_size_lockstep(tmp_path):
+    # 100 x 100 (9.9e7 synapses): first 120 steps including the full sweep of
+    # the first learning-rate switch, compared bit for bit at the end
+    spec = configs.config1()
+    gpu, ora = _lockstep_and_compare(spec, tmp_path, max_steps=120)
+    w = gpu.work()
+    assert w[2] == 100 * 9900 * 100  # one sweep over every synapse
+    gpu.close()
