@@ -32,6 +32,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -46,6 +48,10 @@ def workload(name: str):
     if name == "100x100":
         return configs.config1(), ("lazy BCPNN update, 100 HCU x 100 MCU, dense recurrent "
                                    "projection (9.9e7 synapses), plain FP32 state")
+    if name == "batch512":
+        return configs.batched_nets(512), ("lazy BCPNN update, 512 independent networks "
+                                           "(HCU 8-64, MCU 16-128, sparse recurrent p in "
+                                           "{0.1,0.5,0.9}), plain FP32 state, lock-step")
     if name == "10x10":
         return configs.config0(), ("lazy BCPNN update, 10 HCU x 10 MCU, dense recurrent "
                                    "projection (9e3 synapses), plain FP32 state")
@@ -100,6 +106,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-only", action="store_true")
+    ap.add_argument("--step-times", type=int, default=0,
+                    help="after the timed region, time this many synchronised single steps")
     args = ap.parse_args()
     if args.cpu_only:
         return cpu_only(args)
@@ -115,6 +123,7 @@ def main():
         path = configs.write(spec, d)
         t_init = time.perf_counter()
         s = Solver(path)
+        configs.apply_conn_arrays(s, spec)
         s.sync()
         t_init = time.perf_counter() - t_init
         s.step(args.warmup)
@@ -128,6 +137,17 @@ def main():
         t1 = time.perf_counter()
         launches, ms = s.timing(0)
         w1 = s.work()
+        step_ms = None
+        if args.step_times:
+            # per-step wall times (each step synchronised) for the p50/p99 report
+            ts = []
+            for _ in range(args.step_times):
+                a0 = time.perf_counter()
+                s.step(1)
+                s.sync()
+                ts.append(1e3 * (time.perf_counter() - a0))
+            step_ms = {"p50": float(np.percentile(ts, 50)), "p99": float(np.percentile(ts, 99)),
+                       "n": len(ts)}
         s.close()
     el = t1 - t0
     rows = int(w1[0] - w0[0])
@@ -157,8 +177,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": desc, "hcu": spec.pops[0].hcu, "mcu": spec.pops[0].mcu,
-                   "synapses": spec.pops[0].hcu * int(spec.projs[0].slot_num) * spec.pops[0].mcu,
+        "config": {"workload": desc, "networks": len(spec.pops),
+                   "hcu": spec.total_hcu, "units": spec.total_units,
+                   "synapses": sum(p.hcu * int(q.slot_num) * p.mcu
+                                   for p, q in zip(spec.pops, spec.projs)),
                    "phase": "training steps %d..%d" % (args.warmup + 1, args.warmup + done)},
         "steps_per_sec": done / el,
         "work": {"row_synapse_updates": rows, "col_synapse_updates": cols,
@@ -169,6 +191,10 @@ def main():
                      "kernel": "rowcol_kernel", "avg_launch_ms": avg_ms,
                      "launches": rc_launch, "bytes_per_launch": bytes_per_launch},
     }
+    if step_ms is not None:
+        line["step_ms"] = step_ms
+    if len(spec.pops) > 1:
+        line["network_steps_per_sec"] = len(spec.pops) * done / el
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line))

@@ -51,6 +51,7 @@ def run_lockstep(spec, tmp_path, max_steps=None, check_every=0):
     Returns (solver, oracle_net, first_divergent_step or None, steps)."""
     path = write_spec(spec, tmp_path)
     gpu = Solver(path)
+    configs.apply_conn_arrays(gpu, spec)
     ora = O.OracleNet(spec)
     diverged = None
     steps = 0

@@ -107,3 +107,14 @@ def test_config1_size_lockstep(tmp_path):
     w = gpu.work()
     assert w[2] == 100 * 9900 * 100  # one sweep over every synapse
     gpu.close()
+
+
+def test_batched_networks_bitwise(tmp_path):
+    # independent networks of different sizes and connection densities in one
+    # batched population launch and one batched row/column launch
+    spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=(3, 17), n_pat=4, train=40,
+                                quiet=5, recall=30, reset=20)
+    gpu, ora = _lockstep_and_compare(spec, tmp_path)
+    c, t, flags = gpu.check()
+    assert (c, t) == (ora.check.correct, ora.check.total)
+    gpu.close()
