@@ -196,9 +196,10 @@ def device_math(op: int, inputs: np.ndarray) -> np.ndarray:
     """Evaluate the device's deterministic math on the GPU (tests):
     op 0 expf, 1 logf (uint32 bit patterns in/out), 2 Philox draw
     (rows of seed_lo, seed_hi, step, unit -> u, z bits), 3 float division,
-    4 Wij of (pij, pi, pj)."""
+    4 Wij of (pij, pi, pj), 5 float -> bfloat16 code, 6 float -> binary16 code,
+    7 binary16 code -> float."""
     a = np.ascontiguousarray(inputs, dtype=np.uint32)
-    in_w = {0: 1, 1: 1, 2: 4, 3: 2, 4: 3}[op]
+    in_w = {0: 1, 1: 1, 2: 4, 3: 2, 4: 3, 5: 1, 6: 1, 7: 1}[op]
     out_w = 2 if op == 2 else 1
     n = a.size // in_w
     out = np.zeros(n * out_w, np.uint32)

@@ -100,6 +100,7 @@ class NetSpec:
     # per projection (ii, di) arrays of shape [H*C], written through the
     # Database API instead of a connection-map file (no ProcInitConn)
     conn_arrays: Optional[List[Tuple[np.ndarray, np.ndarray]]] = None
+    trace_precision: str = "fp32"  # storage of Pij/Eij: fp32, bf16 or fp16
 
     @property
     def total_hcu(self) -> int:
@@ -249,6 +250,8 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
     L.append('  argi { key: "wta-pop-max" val: %d }' % spec.wta_pop_max)
     if ext_spike_path:
         L.append('  args { key: "external spike" val: "%s" }' % ext_spike_path)
+    if spec.trace_precision != "fp32":
+        L.append('  args { key: "trace-precision" val: "%s" }' % spec.trace_precision)
     L.append("}")
     if spec.conn_arrays is None:
         L.append("proc_param {")

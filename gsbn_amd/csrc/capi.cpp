@@ -10,6 +10,7 @@
 #include "coeff.h"
 #include "detmath.h"
 #include "framework.h"
+#include "gsbn_core.h"
 #include "philox.h"
 
 namespace gsbn {
@@ -240,6 +241,12 @@ float gsbn_host_expf(float x) { return gsbn_expf(x); }
 float gsbn_host_logf(float x) { return gsbn_logf(x); }
 void gsbn_host_draw(uint64_t seed, uint32_t pop, uint32_t step, uint32_t unit, float* u, float* z) {
   gsbn_draw_unit(seed, pop, step, unit, u, z);
+}
+uint16_t gsbn_host_pack_trace(int prec, float x, float scale) {
+  return gsbn_pack_trace(prec, x, scale);
+}
+float gsbn_host_unpack_trace(int prec, uint16_t h, float inv_scale) {
+  return gsbn_unpack_trace(prec, h, inv_scale);
 }
 
 }  // extern "C"

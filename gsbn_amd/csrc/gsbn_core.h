@@ -90,6 +90,79 @@ GSBN_HD GSBN_INLINE float gsbn_wij(float pij, float pi, float pj, float wgain, f
   return wgain * gsbn_logf((pij + eps2) / ((pi + eps) * (pj + eps)));
 }
 
+/* ---- reduced-precision trace storage ----
+ * float -> bfloat16 / binary16 with round-to-nearest-even (subnormals kept,
+ * overflow to infinity, NaN stays NaN), written with integer operations so
+ * that host and device round identically. */
+GSBN_HD GSBN_INLINE uint16_t gsbn_f32_to_bf16(float x) {
+  uint32_t u = gsbn_fbits(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+GSBN_HD GSBN_INLINE float gsbn_bf16_to_f32(uint16_t h) { return gsbn_bitsf((uint32_t)h << 16); }
+
+GSBN_HD GSBN_INLINE uint16_t gsbn_f32_to_f16(float x) {
+  const uint32_t u = gsbn_fbits(x);
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const uint32_t a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);          /* NaN */
+  if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);         /* >= 65520: inf */
+  if (a < 0x38800000u) {                                            /* below 2^-14: subnormal */
+    if (a < 0x33000000u) return (uint16_t)sign;                     /* < 2^-25: zero */
+    const uint32_t e = a >> 23;
+    const uint32_t m = (a & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126u - e;                                /* 14..24 */
+    uint32_t q = m >> shift;
+    const uint32_t rem = m & ((1u << shift) - 1u);
+    const uint32_t half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t r = a - 0x38000000u;                                     /* rebias exponent */
+  r += 0xfffu + ((r >> 13) & 1u);
+  return (uint16_t)(sign | (r >> 13));
+}
+
+GSBN_HD GSBN_INLINE float gsbn_f16_to_f32(uint16_t h) {
+  const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+  const uint32_t e = ((uint32_t)h >> 10) & 0x1fu;
+  const uint32_t m = (uint32_t)h & 0x3ffu;
+  if (e == 0) {
+    if (m == 0) return gsbn_bitsf(sign);
+    /* subnormal: m * 2^-24, exact in float */
+    const float v = (float)m * 5.9604644775390625e-08f;
+    return sign ? -v : v;
+  }
+  if (e == 31) return gsbn_bitsf(sign | 0x7f800000u | (m << 13));
+  return gsbn_bitsf(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+/* 16-byte synapse record of the reduced-precision forms; wij is kept in a
+ * separate float array because only row updates touch it */
+typedef struct {
+  uint16_t pij, eij;
+  float zi2, zj2;
+  int32_t tij;
+} gsbn_syn16_t;
+
+#ifndef GSBN_PREC_F32
+#define GSBN_PREC_F32 0
+#define GSBN_PREC_BF16 1
+#define GSBN_PREC_F16 2
+#endif
+
+/* stored value of a P or E trace: bf16(x), or binary16(x * scale) with a
+ * power-of-two scale (so that x * scale and the decode's product with
+ * 1 / scale are exact) */
+GSBN_HD GSBN_INLINE uint16_t gsbn_pack_trace(int prec, float x, float scale) {
+  return prec == GSBN_PREC_BF16 ? gsbn_f32_to_bf16(x) : gsbn_f32_to_f16(x * scale);
+}
+GSBN_HD GSBN_INLINE float gsbn_unpack_trace(int prec, uint16_t h, float inv_scale) {
+  return prec == GSBN_PREC_BF16 ? gsbn_bf16_to_f32(h) : gsbn_f16_to_f32(h) * inv_scale;
+}
+
 /* table lookup: gaps beyond the table use the last entry, which the builder
  * only allows when that entry is the exact saturated value */
 GSBN_HD GSBN_INLINE const float* gsbn_coef(const float* table, int len, int gap) {

@@ -30,6 +30,8 @@
 static_assert(sizeof(gsbn_syn_rec) == 32, "synapse record must be 32 bytes");
 static_assert(sizeof(gsbn_syn_t) == sizeof(gsbn_syn_rec), "record layouts differ");
 static_assert(sizeof(gsbn_row_rec) == 16, "row record must be 16 bytes");
+static_assert(sizeof(gsbn_syn16_rec) == 16 && sizeof(gsbn_syn16_t) == 16,
+              "16-bit synapse record must be 16 bytes");
 
 #define POP_MAX_THREADS 1024
 #define RC_THREADS 256
@@ -349,15 +351,113 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_arg
   if (j < a.nproj_in) a.projs[j].pend_cnt[h] = 0;
 }
 
-__device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const int h, const int part) {
+/* ---- synapse state access for the FP32 record and the 16-bit forms ---- */
+struct SynVals {
+  float pij, eij, zi2, zj2, wij;
+  int tij;
+};
+
+template <int PREC>
+struct SynIO {
+  gsbn_syn_rec* syn;
+  gsbn_syn16_rec* syn16;
+  float* wij;
+  float ps, es, pinv, einv;
+
+  template <class A>
+  __device__ __forceinline__ explicit SynIO(const A& a)
+      : syn(a.syn), syn16(a.syn16), wij(a.wij), ps(a.p_scale), es(a.e_scale) {
+    pinv = PREC == GSBN_PREC_F16 ? 1.0f / ps : 1.0f;
+    einv = PREC == GSBN_PREC_F16 ? 1.0f / es : 1.0f;
+  }
+
+  __device__ __forceinline__ void unpack16(const uint4 v, SynVals& s) const {
+    s.pij = gsbn_unpack_trace(PREC, (uint16_t)(v.x & 0xffffu), pinv);
+    s.eij = gsbn_unpack_trace(PREC, (uint16_t)(v.x >> 16), einv);
+    s.zi2 = __uint_as_float(v.y);
+    s.zj2 = __uint_as_float(v.z);
+    s.tij = (int)v.w;
+  }
+  __device__ __forceinline__ uint4 pack16(const SynVals& s) const {
+    const uint32_t p = gsbn_pack_trace(PREC, s.pij, ps);
+    const uint32_t e = gsbn_pack_trace(PREC, s.eij, es);
+    return make_uint4(p | (e << 16), __float_as_uint(s.zi2), __float_as_uint(s.zj2), (uint32_t)s.tij);
+  }
+
+  /* whole state; the weight is read only when need_w (16-bit forms) */
+  __device__ __forceinline__ SynVals load(int64_t i, bool need_w) const {
+    SynVals s;
+    if constexpr (PREC == GSBN_PREC_F32) {
+      const gsbn_syn_rec r = syn[i];
+      s.pij = r.pij; s.eij = r.eij; s.zi2 = r.zi2; s.zj2 = r.zj2; s.wij = r.wij; s.tij = r.tij;
+    } else {
+      unpack16(*reinterpret_cast<const uint4*>(syn16 + i), s);
+      s.wij = need_w ? wij[i] : 0.0f;
+    }
+    return s;
+  }
+  /* whole state; the weight is written only when w_new (16-bit forms) */
+  __device__ __forceinline__ void store(int64_t i, const SynVals& s, bool w_new) const {
+    if constexpr (PREC == GSBN_PREC_F32) {
+      gsbn_syn_rec r;
+      r.pij = s.pij; r.eij = s.eij; r.zi2 = s.zi2; r.zj2 = s.zj2; r.wij = s.wij; r.tij = s.tij;
+      r.pad0 = 0; r.pad1 = 0;
+      syn[i] = r;
+    } else {
+      *reinterpret_cast<uint4*>(syn16 + i) = pack16(s);
+      if (w_new) wij[i] = s.wij;
+    }
+  }
+  /* the traces and tij only (column updates never touch the weight) */
+  __device__ __forceinline__ SynVals load_traces(int64_t i) const {
+    SynVals s;
+    if constexpr (PREC == GSBN_PREC_F32) {
+      const float4 v = *reinterpret_cast<const float4*>(syn + i);
+      s.pij = v.x; s.eij = v.y; s.zi2 = v.z; s.zj2 = v.w; s.wij = 0.0f;
+      s.tij = syn[i].tij;
+    } else {
+      unpack16(*reinterpret_cast<const uint4*>(syn16 + i), s);
+      s.wij = 0.0f;
+    }
+    return s;
+  }
+  __device__ __forceinline__ void store_traces(int64_t i, const SynVals& s) const {
+    if constexpr (PREC == GSBN_PREC_F32) {
+      *reinterpret_cast<float4*>(syn + i) = make_float4(s.pij, s.eij, s.zi2, s.zj2);
+      syn[i].tij = s.tij;
+    } else {
+      *reinterpret_cast<uint4*>(syn16 + i) = pack16(s);
+    }
+  }
+  __device__ __forceinline__ void store_tij(int64_t i, int t) const {
+    if constexpr (PREC == GSBN_PREC_F32)
+      syn[i].tij = t;
+    else
+      syn16[i].tij = t;
+  }
+};
+
+/* per-launch values of a row/column update (taken from the argument struct
+ * in the single form, from the step arguments in the batched form) */
+struct RcStep {
+  int t, exec, P, table_len;
+  float kp;
+  const float* table;
+};
+
+template <int PREC>
+__device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcStep& q,
+                                            const int h, const int part) {
   __shared__ float sh_pi[RC_THREADS];
   __shared__ int sh_a[RC_THREADS];
   __shared__ int sh_r[RC_THREADS];
 
-  const int P = a.parts;
+  const SynIO<PREC> io(a);
+  const int P = q.P;
   const int M = a.M, C = a.C;
   const int tid = threadIdx.x;
-  const int t = a.simstep;
+  const int t = q.t;
+  const bool learn = q.kp != 0.0f;
 
   /* rows part, part+P, ... of this HCU's ascending active-row list */
   const int nrows = a.rows_cnt[h];
@@ -375,7 +475,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const int
         st.zi += a.kfti;
         st.ti = t;
       } else {
-        const float* c = gsbn_coef(a.table, a.table_len, D);
+        const float* c = gsbn_coef(q.table, q.table_len, D);
         gsbn_row_catchup(c, &st.pi, &st.ei, &st.zi);
         st.zi = st.zi + a.kfti;
         st.ti = t;
@@ -390,31 +490,29 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const int
       const int rr = f / M;
       const int j = f - rr * M;
       const int64_t sidx = (int64_t)sh_r[rr] * M + j;
-      gsbn_syn_rec s = a.syn[sidx];
+      SynVals s = io.load(sidx, !learn);
       float contrib;
+      bool w_new = false;
       const int D = t - s.tij;
       if (D <= 0) {
         s.zi2 += a.kfti;
         s.tij = t;
         contrib = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
       } else {
-        const float* c = gsbn_coef(a.table, a.table_len, D);
+        const float* c = gsbn_coef(q.table, q.table_len, D);
         gsbn_syn_catchup(c, &s.pij, &s.eij, &s.zi2, &s.zj2);
         s.zi2 = s.zi2 + a.kfti;
         s.tij = t;
-        float w;
-        if (a.kp != 0.0f) {
-          w = gsbn_wij(s.pij, sh_pi[rr], a.pj[h * M + j], a.wgain, a.eps, a.eps2);
-          s.wij = w;
-        } else {
-          w = s.wij;
+        if (learn) {
+          s.wij = gsbn_wij(s.pij, sh_pi[rr], a.pj[h * M + j], a.wgain, a.eps, a.eps2);
+          w_new = true;
         }
-        contrib = w;
+        contrib = s.wij;
       }
       /* column increment of a spiking postsynaptic unit after the row
        * update (the reference's col pass sees D == 0 here) */
       if (a.sj[h * M + j] > 0) s.zj2 += a.kftj;
-      a.syn[sidx] = s;
+      io.store(sidx, s, w_new);
       a.contrib[((int64_t)h * C + sh_a[rr]) * M + j] = contrib;
     }
     __syncthreads();
@@ -430,50 +528,50 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const int
     const int j = a.spk_list[h * M + s_i];
     for (int k = k0 + tid; k < k1; k += RC_THREADS) {
       const int r = h * C + k;
-      if (a.ii[r] < 0 || a.act_stamp[r] == a.exec) continue;
-      gsbn_syn_rec* sp = a.syn + (int64_t)r * M + j;
-      float4 v = *reinterpret_cast<float4*>(sp);
-      const int tij = sp->tij;
-      float pij = v.x, eij = v.y, zi2 = v.z, zj2 = v.w;
-      const int D = t - tij;
+      if (a.ii[r] < 0 || a.act_stamp[r] == q.exec) continue;
+      const int64_t sidx = (int64_t)r * M + j;
+      SynVals v = io.load_traces(sidx);
+      const int D = t - v.tij;
       if (D <= 0) {
-        zj2 += a.kftj;
+        v.zj2 += a.kftj;
       } else {
-        const float* c = gsbn_coef(a.table, a.table_len, D);
-        gsbn_syn_catchup(c, &pij, &eij, &zi2, &zj2);
-        zj2 = zj2 + a.kftj;
+        const float* c = gsbn_coef(q.table, q.table_len, D);
+        gsbn_syn_catchup(c, &v.pij, &v.eij, &v.zi2, &v.zj2);
+        v.zj2 = v.zj2 + a.kftj;
       }
-      *reinterpret_cast<float4*>(sp) = make_float4(pij, eij, zi2, zj2);
-      sp->tij = t;
+      v.tij = t;
+      io.store_traces(sidx, v);
     }
   }
 }
 
+template <int PREC>
 __global__ void __launch_bounds__(RC_THREADS) rowcol_kernel(gsbn_rowcol_args a) {
   const int h = blockIdx.x / a.parts;
-  rowcol_body(a, h, blockIdx.x - h * a.parts);
+  const RcStep q = {a.simstep, a.exec, a.parts, a.table_len, a.kp, a.table};
+  rowcol_body<PREC>(a, q, h, blockIdx.x - h * a.parts);
 }
 
+template <int PREC>
 __global__ void __launch_bounds__(RC_THREADS)
     rowcol_kernel_batch(const gsbn_rowcol_args* __restrict__ arr, const int32_t* __restrict__ wg_proj,
                         const int32_t* __restrict__ wg_hcu, int parts, gsbn_step_args st) {
   const int w = blockIdx.x / parts;
-  gsbn_rowcol_args a = arr[wg_proj[w]];
-  a.simstep = st.simstep;
-  a.exec = st.exec;
-  a.kp = a.taupdt * st.prn;
-  a.table = a.tables[st.kp_slot];
-  a.table_len = a.table_lens[st.kp_slot];
-  a.parts = parts;
-  rowcol_body(a, wg_hcu[w], blockIdx.x - w * parts);
+  const gsbn_rowcol_args& a = arr[wg_proj[w]];
+  const RcStep q = {st.simstep, st.exec, parts, a.table_lens[st.kp_slot], a.taupdt * st.prn,
+                    a.tables[st.kp_slot]};
+  rowcol_body<PREC>(a, q, wg_hcu[w], blockIdx.x - w * parts);
 }
 
+template <int PREC>
 __global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
   __shared__ float sh_pi[ALL_ROWS];
+  const SynIO<PREC> io(a);
   const int64_t R = (int64_t)a.H * a.C;
   const int M = a.M;
   const int tid = threadIdx.x;
   const int t = a.target;
+  const bool learn = a.kp != 0.0f;
   const int64_t nchunks = (R + ALL_ROWS - 1) / ALL_ROWS;
   for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
     const int64_t r0 = ch * ALL_ROWS;
@@ -495,20 +593,20 @@ __global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
       const int rr = f / M;
       const int j = f - rr * M;
       const int64_t r = r0 + rr;
-      gsbn_syn_rec* sp = a.syn + r * M + j;
-      gsbn_syn_rec s = *sp;
+      const int64_t sidx = r * M + j;
+      SynVals s = io.load(sidx, false);
       const int D = t - s.tij;
       if (D <= 0) {
-        sp->tij = t;
+        io.store_tij(sidx, t);
       } else {
         const float* c = gsbn_coef(a.table, a.table_len, D);
         gsbn_syn_catchup(c, &s.pij, &s.eij, &s.zi2, &s.zj2);
         s.tij = t;
-        if (a.kp != 0.0f) {
+        if (learn) {
           const int h = (int)(r / a.C);
           s.wij = gsbn_wij(s.pij, sh_pi[rr], a.pj[h * M + j], a.wgain, a.eps, a.eps2);
         }
-        *sp = s;
+        io.store(sidx, s, learn);
       }
     }
     __syncthreads();
@@ -640,6 +738,44 @@ __global__ void i32_fill_kernel(int32_t* p, int64_t n, int32_t v) {
     p[i] = v;
 }
 
+__global__ void syn16_field_kernel(gsbn_syn16_rec* syn, float* wij, int64_t n, int field,
+                                   uint32_t* buf, int to_records, int prec, float ps, float es) {
+  const float pinv = prec == GSBN_PREC_F16 ? 1.0f / ps : 1.0f;
+  const float einv = prec == GSBN_PREC_F16 ? 1.0f / es : 1.0f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    gsbn_syn16_rec* r = syn + i;
+    if (to_records) {
+      const uint32_t v = buf[i];
+      switch (field) {
+        case 0: r->pij = gsbn_pack_trace(prec, gsbn_bitsf(v), ps); break;
+        case 1: r->eij = gsbn_pack_trace(prec, gsbn_bitsf(v), es); break;
+        case 2: r->zi2 = gsbn_bitsf(v); break;
+        case 3: r->zj2 = gsbn_bitsf(v); break;
+        case 4: wij[i] = gsbn_bitsf(v); break;
+        default: r->tij = (int32_t)v; break;
+      }
+    } else {
+      uint32_t v;
+      switch (field) {
+        case 0: v = gsbn_fbits(gsbn_unpack_trace(prec, r->pij, pinv)); break;
+        case 1: v = gsbn_fbits(gsbn_unpack_trace(prec, r->eij, einv)); break;
+        case 2: v = gsbn_fbits(r->zi2); break;
+        case 3: v = gsbn_fbits(r->zj2); break;
+        case 4: v = gsbn_fbits(wij[i]); break;
+        default: v = (uint32_t)r->tij; break;
+      }
+      buf[i] = v;
+    }
+  }
+}
+
+__global__ void syn16_fill_kernel(gsbn_syn16_rec* syn, int64_t n, gsbn_syn16_rec v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    syn[i] = v;
+}
+
 __global__ void device_math_kernel(int op, int64_t n, const uint32_t* in, uint32_t* out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -658,6 +794,12 @@ __global__ void device_math_kernel(int op, int64_t n, const uint32_t* in, uint32
     } else if (op == 4) {
       out[i] = gsbn_fbits(gsbn_wij(gsbn_bitsf(in[3 * i]), gsbn_bitsf(in[3 * i + 1]),
                                    gsbn_bitsf(in[3 * i + 2]), 1.0f, 1e-3f, 1e-6f));
+    } else if (op == 5) {
+      out[i] = gsbn_f32_to_bf16(gsbn_bitsf(in[i]));
+    } else if (op == 6) {
+      out[i] = gsbn_f32_to_f16(gsbn_bitsf(in[i]));
+    } else if (op == 7) {
+      out[i] = gsbn_fbits(gsbn_f16_to_f32((uint16_t)in[i]));
     }
   }
 }
@@ -670,6 +812,23 @@ int grid_for(int64_t n, int threads, int cap) {
 }
 
 }  // namespace
+
+/* power of two, normal, positive: multiplying by it and by its inverse is exact */
+static bool pow2_scale(float x) {
+  const uint32_t b = gsbn_fbits(x);
+  return (b & 0x807fffffu) == 0 && (b >> 23) != 0 && (b >> 23) != 255;
+}
+
+template <class A>
+static bool trace_form_ok(const A& a) {
+  switch (a.prec) {
+    case GSBN_PREC_F32: return a.syn != nullptr || (int64_t)a.H * a.C == 0;
+    case GSBN_PREC_BF16: return a.syn16 != nullptr && a.wij != nullptr;
+    case GSBN_PREC_F16:
+      return a.syn16 != nullptr && a.wij != nullptr && pow2_scale(a.p_scale) && pow2_scale(a.e_scale);
+    default: return false;
+  }
+}
 
 extern "C" {
 
@@ -701,11 +860,24 @@ int gsbn_pop_flush(const gsbn_pop_args* a, void* stream) {
 }
 
 int gsbn_proj_all(const gsbn_all_args* a, void* stream) {
-  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2) return GSBN_ERR_ARG;
+  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2 ||
+      !trace_form_ok(*a))
+    return GSBN_ERR_ARG;
   const int64_t R = (int64_t)a->H * a->C;
   if (R == 0) return 0;
   const int grid = grid_for((R + ALL_ROWS - 1) / ALL_ROWS, 1, 8192);
-  hipLaunchKernelGGL(all_kernel, dim3(grid), dim3(ALL_THREADS), 0, (hipStream_t)stream, *a);
+  hipStream_t s = (hipStream_t)stream;
+  switch (a->prec) {
+    case GSBN_PREC_F32:
+      hipLaunchKernelGGL(all_kernel<GSBN_PREC_F32>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      break;
+    case GSBN_PREC_BF16:
+      hipLaunchKernelGGL(all_kernel<GSBN_PREC_BF16>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      break;
+    default:
+      hipLaunchKernelGGL(all_kernel<GSBN_PREC_F16>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      break;
+  }
   return launch_status();
 }
 
@@ -718,7 +890,8 @@ int gsbn_proj_jxx(const gsbn_jxx_args* a, void* stream) {
 }
 
 int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream) {
-  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2 || a->parts < 0)
+  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2 || a->parts < 0 ||
+      !trace_form_ok(*a))
     return GSBN_ERR_ARG;
   gsbn_rowcol_args b = *a;
   if (b.parts == 0) {
@@ -727,8 +900,18 @@ int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream) {
   }
   const int64_t grid = (int64_t)b.H * b.parts;
   if (grid > 0x7fffffff) return GSBN_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(rowcol_kernel, dim3((unsigned)grid), dim3(RC_THREADS), 0,
-                     (hipStream_t)stream, b);
+  hipStream_t s = (hipStream_t)stream;
+  switch (b.prec) {
+    case GSBN_PREC_F32:
+      hipLaunchKernelGGL(rowcol_kernel<GSBN_PREC_F32>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
+      break;
+    case GSBN_PREC_BF16:
+      hipLaunchKernelGGL(rowcol_kernel<GSBN_PREC_BF16>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
+      break;
+    default:
+      hipLaunchKernelGGL(rowcol_kernel<GSBN_PREC_F16>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
+      break;
+  }
   return launch_status();
 }
 
@@ -756,12 +939,27 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
                            const int32_t* wg_hcu, int32_t nhcu, int32_t parts,
                            const gsbn_step_args* step, void* stream) {
   if (!projs_dev || !wg_proj || !wg_hcu || !step || nhcu <= 0 || parts <= 0 ||
-      step->kp_slot < 0 || step->kp_slot >= GSBN_KP_SLOTS)
+      step->kp_slot < 0 || step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 ||
+      step->prec > GSBN_PREC_F16)
     return GSBN_ERR_ARG;
   const int64_t grid = (int64_t)nhcu * parts;
   if (grid > 0x7fffffff) return GSBN_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(rowcol_kernel_batch, dim3((unsigned)grid), dim3(RC_THREADS), 0,
-                     (hipStream_t)stream, projs_dev, wg_proj, wg_hcu, parts, *step);
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 g((unsigned)grid), b(RC_THREADS);
+  switch (step->prec) {
+    case GSBN_PREC_F32:
+      hipLaunchKernelGGL(rowcol_kernel_batch<GSBN_PREC_F32>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+                         parts, *step);
+      break;
+    case GSBN_PREC_BF16:
+      hipLaunchKernelGGL(rowcol_kernel_batch<GSBN_PREC_BF16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+                         parts, *step);
+      break;
+    default:
+      hipLaunchKernelGGL(rowcol_kernel_batch<GSBN_PREC_F16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+                         parts, *step);
+      break;
+  }
   return launch_status();
 }
 
@@ -838,6 +1036,27 @@ int gsbn_row_fill(gsbn_row_rec* row, int64_t n, gsbn_row_rec v, void* stream) {
   return launch_status();
 }
 
+int gsbn_syn16_field(gsbn_syn16_rec* syn, float* wij, int64_t n, int field, void* buf,
+                     int to_records, int prec, float p_scale, float e_scale, void* stream) {
+  if (n < 0 || field < 0 || field > 5 || prec < GSBN_PREC_BF16 || prec > GSBN_PREC_F16 ||
+      (n && (!syn || !buf || (field == 4 && !wij))) ||
+      (prec == GSBN_PREC_F16 && (!pow2_scale(p_scale) || !pow2_scale(e_scale))))
+    return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(syn16_field_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
+                     (hipStream_t)stream, syn, wij, n, field, (uint32_t*)buf, to_records, prec,
+                     p_scale, e_scale);
+  return launch_status();
+}
+
+int gsbn_syn16_fill(gsbn_syn16_rec* syn, int64_t n, gsbn_syn16_rec v, void* stream) {
+  if (n < 0 || (n && !syn)) return GSBN_ERR_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(syn16_fill_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
+                     (hipStream_t)stream, syn, n, v);
+  return launch_status();
+}
+
 int gsbn_i32_fill(int32_t* p, int64_t n, int32_t v, void* stream) {
   if (n < 0 || (n && !p)) return GSBN_ERR_ARG;
   if (n == 0) return 0;
@@ -847,7 +1066,7 @@ int gsbn_i32_fill(int32_t* p, int64_t n, int32_t v, void* stream) {
 }
 
 int gsbn_device_math(int op, int64_t n, const void* in_host, void* out_host) {
-  if (op < 0 || op > 4 || n < 0) return GSBN_ERR_ARG;
+  if (op < 0 || op > 7 || n < 0) return GSBN_ERR_ARG;
   if (n == 0) return 0;
   const int in_w = op == 2 ? 4 : op == 3 ? 2 : op == 4 ? 3 : 1;
   const int out_w = op == 2 ? 2 : 1;

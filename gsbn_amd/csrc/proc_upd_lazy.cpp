@@ -169,7 +169,22 @@ class Proj {
                    *_wij = nullptr;
   vector<VecBase*> views;
   /* device state */
-  DevBuf<gsbn_syn_rec> syn;
+  int prec = GSBN_PREC_F32;     /* trace form of pij/eij */
+  int p_shift = 0, e_shift = 0; /* FP16 form: stored value is binary16(x * 2^shift) */
+  int64_t nsyn = 0;
+  DevBuf<gsbn_syn_rec> syn;     /* FP32 form */
+  DevBuf<gsbn_syn16_rec> syn16; /* 16-bit forms */
+  DevBuf<float> wij16;          /* 16-bit forms: the weights */
+  float p_scale() const { return std::ldexp(1.0f, p_shift); }
+  float e_scale() const { return std::ldexp(1.0f, e_shift); }
+  void syn_field(int field, void* buf, int to_records) {
+    if (prec == GSBN_PREC_F32)
+      op_check(gsbn_syn_field(syn.p, nsyn, field, buf, to_records, active_stream()), "gsbn_syn_field");
+    else
+      op_check(gsbn_syn16_field(syn16.p, wij16.p, nsyn, field, buf, to_records, prec, p_scale(),
+                                e_scale(), active_stream()),
+               "gsbn_syn16_field");
+  }
   DevBuf<gsbn_row_rec> rowst;
   DevBuf<float> contrib;
   DevBuf<int32_t> rows_list, rows_cnt, pend_cnt, act_stamp, hist_list, hist_cnt, inv_off, inv_slot;
@@ -200,17 +215,17 @@ struct SynField : VecView {
   int field;
   SynField(Proj* q, int f) : p(q), field(f) {}
   void pull(VecBase* v) override {
-    const size_t n = p->syn.n;
+    const size_t n = (size_t)p->nsyn;
     DevBuf<uint32_t> tmp;
     tmp.alloc(n);
-    op_check(gsbn_syn_field(p->syn.p, (int64_t)n, field, tmp.p, 0, active_stream()), "gsbn_syn_field");
+    p->syn_field(field, tmp.p, 0);
     tmp.download((uint32_t*)raw_data(v), n);
   }
   void push(VecBase* v) override {
-    const size_t n = p->syn.n;
+    const size_t n = (size_t)p->nsyn;
     DevBuf<uint32_t> tmp;
     tmp.upload((const uint32_t*)raw_data(v), n);
-    op_check(gsbn_syn_field(p->syn.p, (int64_t)n, field, tmp.p, 1, active_stream()), "gsbn_syn_field");
+    p->syn_field(field, tmp.p, 1);
     hip_check(hipStreamSynchronize(active_stream()), "sync");
   }
 };
@@ -519,9 +534,27 @@ void Pop::flush() {
 
 /* ---- Proj ---- */
 
+/* Power-of-two scale of the FP16 form.  A presynaptic trace gains kfti per
+ * spike (at most one per step) and decays by exp(-kzi) per step, so
+ * zi2 <= kfti / (1 - exp(-kzi)), likewise zj2; Eij and Pij are averages of
+ * zi2*zj2 (and of the initial Pij <= 1), so they never exceed the product
+ * of the two limits.  The largest 2^k that keeps this bound below the
+ * binary16 maximum 65504 maps every reachable value to a finite number and
+ * leaves as much range as possible to small values. */
+static int fp16_shift(float kzi, float kzj, float kfti, float kftj) {
+  const double zi = (double)kfti / -std::expm1(-(double)kzi);
+  const double zj = (double)kftj / -std::expm1(-(double)kzj);
+  const double bound = std::max(1.0, zi * zj);
+  if (!std::isfinite(bound) || kzi <= 0 || kzj <= 0)
+    throw Error(-2, "ProcUpdLazy: fp16 trace form needs finite positive z decay rates");
+  int k = (int)std::floor(std::log2(65504.0 / bound));
+  while (k > -126 && bound * std::ldexp(1.0, k) > 65504.0) --k; /* guard log2 rounding */
+  if (k < -100) throw Error(-2, "ProcUpdLazy: fp16 trace form: trace bound out of range");
+  return std::min(k, 100);
+}
+
 void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
                     vector<Proj*>* list_proj, vector<Pop*>* list_pop) {
-  (void)proc_param;
   GlobalVar glv;
   _id = (int)list_proj->size();
   list_proj->push_back(this);
@@ -594,12 +627,46 @@ void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
   _si->resize(_nsrc);
 
   /* device state, initial values of Proj.cpp:88-105 */
-  syn.alloc((size_t)S);
+  Parser par(proc_param);
+  string form;
+  if (par.args("trace-precision", form)) {
+    if (form == "fp32")
+      prec = GSBN_PREC_F32;
+    else if (form == "bf16")
+      prec = GSBN_PREC_BF16;
+    else if (form == "fp16")
+      prec = GSBN_PREC_F16;
+    else
+      throw Error(-2, "ProcUpdLazy: trace-precision must be fp32, bf16 or fp16, got '" + form + "'");
+  }
+  nsyn = S;
   rowst.alloc((size_t)R);
-  gsbn_syn_rec s0;
-  memset(&s0, 0, sizeof(s0));
-  s0.pij = (float)(1.0 / _dim_conn / _dim_mcu);
-  op_check(gsbn_syn_fill(syn.p, S, s0, active_stream()), "gsbn_syn_fill");
+  const float pij0 = (float)(1.0 / _dim_conn / _dim_mcu);
+  if (prec == GSBN_PREC_F32) {
+    syn.alloc((size_t)S);
+    gsbn_syn_rec s0;
+    memset(&s0, 0, sizeof(s0));
+    s0.pij = pij0;
+    op_check(gsbn_syn_fill(syn.p, S, s0, active_stream()), "gsbn_syn_fill");
+  } else {
+    if (prec == GSBN_PREC_F16) {
+      p_shift = fp16_shift(_tauzidt, _tauzjdt, _kfti, _kftj);
+      e_shift = p_shift;
+    }
+    syn16.alloc((size_t)S);
+    wij16.alloc((size_t)S);
+    gsbn_syn16_rec s0;
+    memset(&s0, 0, sizeof(s0));
+    s0.pij = gsbn_pack_trace(prec, pij0, p_scale());
+    op_check(gsbn_syn16_fill(syn16.p, S, s0, active_stream()), "gsbn_syn16_fill");
+    op_check(gsbn_i32_fill(reinterpret_cast<int32_t*>(wij16.p), S, 0, active_stream()), "fill");
+  }
+  {
+    GlobalVar g;
+    g.puti("trace-precision-" + sid, prec);
+    g.puti("trace-p-shift-" + sid, p_shift);
+    g.puti("trace-e-shift-" + sid, e_shift);
+  }
   gsbn_row_rec r0;
   memset(&r0, 0, sizeof(r0));
   r0.pi = (float)(1.0 / _dim_conn);
@@ -756,6 +823,11 @@ void Proj::update_all(int simstep, float old_prn) {
   a.syn = syn.p;
   a.rowst = rowst.p;
   a.pj = _pj->gpu_data();
+  a.prec = prec;
+  a.syn16 = syn16.p;
+  a.wij = wij16.p;
+  a.p_scale = p_scale();
+  a.e_scale = e_scale();
   Timing::get().begin(2, active_stream());
   op_check(gsbn_proj_all(&a, active_stream()), "gsbn_proj_all");
   Timing::get().end(2, active_stream());
@@ -811,6 +883,11 @@ gsbn_rowcol_args Proj::static_args(const float* slot_prn, int nslots, int64_t ne
   a.act_stamp = act_stamp.p;
   a.contrib = contrib.p;
   a.taupdt = _taupdt;
+  a.prec = prec;
+  a.syn16 = syn16.p;
+  a.wij = wij16.p;
+  a.p_scale = p_scale();
+  a.e_scale = e_scale();
   for (int i = 0; i < nslots; ++i) {
     const CoefTable& t = table(_taupdt * slot_prn[i], need);
     a.tables[i] = t.dev.p;
@@ -1044,6 +1121,7 @@ void ProcUpdLazy::update_gpu() {
   st.do_jxx = sweep ? 0 : 1;
   st.kp_slot = slot;
   st.prn = prn;
+  st.prec = _list_proj.empty() ? GSBN_PREC_F32 : _list_proj[0]->prec;
   st.lginp_offset = lg_off;
   st.wmask_offset = wm_off;
 

@@ -51,6 +51,20 @@ typedef struct {
   int32_t ti;
 } gsbn_row_rec;
 
+/* Reduced-precision trace forms: Pij and Eij stored in 16 bits, all
+ * arithmetic in float.  GSBN_PREC_BF16 stores bfloat16(x); GSBN_PREC_F16
+ * stores binary16(x * scale) with a power-of-two scale per projection (one
+ * for Pij, one for Eij).  Conversions round to nearest, ties to even.  The
+ * weights live in a separate float array in these forms. */
+#define GSBN_PREC_F32 0
+#define GSBN_PREC_BF16 1
+#define GSBN_PREC_F16 2
+typedef struct {
+  uint16_t pij, eij;
+  float zi2, zj2;
+  int32_t tij;
+} gsbn_syn16_rec;
+
 #define GSBN_HIST 64 /* exec steps kept in the spike queue ring (delays <= 31) */
 #define GSBN_MAX_PROJ_IN 8
 #define GSBN_KP_SLOTS 4 /* distinct learning rates a batched projection can switch between */
@@ -60,7 +74,7 @@ typedef struct {
 typedef struct {
   int32_t simstep, exec, do_jxx, kp_slot;
   float prn;
-  int32_t pad;
+  int32_t prec; /* trace form (GSBN_PREC_*) of every projection of a batched row/column launch */
   int64_t lginp_offset; /* added to every population's lginp pointer */
   int64_t wmask_offset; /* added to every population's wmask pointer */
 } gsbn_step_args;
@@ -151,6 +165,11 @@ typedef struct {
   gsbn_syn_rec* syn;
   gsbn_row_rec* rowst;
   const float* pj;
+  /* reduced-precision form (prec != GSBN_PREC_F32): syn16 + wij replace syn */
+  int32_t prec;
+  gsbn_syn16_rec* syn16;
+  float* wij;
+  float p_scale, e_scale; /* GSBN_PREC_F16 only; powers of two */
 } gsbn_all_args;
 int gsbn_proj_all(const gsbn_all_args* a, void* stream);
 
@@ -193,6 +212,13 @@ typedef struct {
   float taupdt;
   const float* tables[GSBN_KP_SLOTS];
   int32_t table_lens[GSBN_KP_SLOTS];
+  /* reduced-precision form (prec != GSBN_PREC_F32): syn16 + wij replace syn;
+   * Wij is computed from the float value of Pij before it is stored.  All
+   * projections of one batched launch must share prec. */
+  int32_t prec;
+  gsbn_syn16_rec* syn16;
+  float* wij;
+  float p_scale, e_scale; /* GSBN_PREC_F16 only; powers of two */
 } gsbn_rowcol_args;
 int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream);
 
@@ -240,10 +266,20 @@ int gsbn_syn_field(gsbn_syn_rec* syn, int64_t n, int field, void* buf, int to_re
 int gsbn_row_field(gsbn_row_rec* row, int64_t n, int field, void* buf, int to_records,
                    void* stream);
 
+/* the same for the reduced-precision records (field 4 = the wij array);
+ * values are converted to / from float with the form's rounding and scale */
+int gsbn_syn16_field(gsbn_syn16_rec* syn, float* wij, int64_t n, int field, void* buf,
+                     int to_records, int prec, float p_scale, float e_scale, void* stream);
+
 /* fill record / int arrays with one value (state initialisation) */
 int gsbn_syn_fill(gsbn_syn_rec* syn, int64_t n, gsbn_syn_rec v, void* stream);
 int gsbn_row_fill(gsbn_row_rec* row, int64_t n, gsbn_row_rec v, void* stream);
 int gsbn_i32_fill(int32_t* p, int64_t n, int32_t v, void* stream);
+int gsbn_syn16_fill(gsbn_syn16_rec* syn, int64_t n, gsbn_syn16_rec v, void* stream);
+
+/* host copies of the 16-bit trace conversions (same rounding as the kernels) */
+uint16_t gsbn_host_pack_trace(int prec, float x, float scale);
+float gsbn_host_unpack_trace(int prec, uint16_t h, float inv_scale);
 
 /* Coefficient table entry (host function): GSBN_COEF_STRIDE = 12 floats
  * Ezi Ezj Ee Ep Cez_ij Cpe Cpz_ij Cez_i Cpz_i 0 0 0 for an elapsed gap. */
@@ -257,7 +293,9 @@ float gsbn_host_expf(float x);
 float gsbn_host_logf(float x);
 void gsbn_host_draw(uint64_t seed, uint32_t pop, uint32_t step, uint32_t unit, float* u,
                     float* z);
-/* evaluate the device math on the GPU for n inputs (tests) */
+/* evaluate the device math on the GPU for n inputs (tests): op 0 expf,
+ * 1 logf, 2 Philox draw, 3 float division, 4 Wij, 5 float -> bfloat16 code,
+ * 6 float -> binary16 code, 7 binary16 code -> float */
 int gsbn_device_math(int op, int64_t n, const void* in_host, void* out_host);
 
 /* ---- solver level ---- */

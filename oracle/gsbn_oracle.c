@@ -264,6 +264,8 @@ typedef struct {
   void *pi, *ei, *zi, *pj, *ej, *zj, *pij, *eij, *zi2, *zj2, *wij;
   int32_t *ssi, *ssj;
   int nssi, nssj;
+  int store_prec;          /* GOR_STORE_*: rounding of stored Pij / Eij */
+  int p_shift, e_shift;    /* binary16 form: x is stored as binary16(x * 2^shift) */
   int ncache;
   gor_coefcache_t cache[GOR_MAXCACHE];
 } gor_proj_t;
@@ -629,42 +631,101 @@ static void proj_update_que(gor_proj_t* q) {
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* 16-bit storage of Pij / Eij.  Independent of the product's integer
+ * bit manipulation: the value is rounded in double arithmetic to the
+ * format's quantum at its binade with nearbyint (round half to even in the
+ * default rounding mode).
+ *   bfloat16: 8 significant bits, exponent range of float (smallest
+ *             quantum 2^-133, overflow above the largest finite value);
+ *   binary16: 11 significant bits of x * 2^shift, smallest quantum 2^-24,
+ *             results above 65504 overflow to infinity. */
+float gor_round_trace(int prec, float x, int shift) {
+  if (prec == GOR_STORE_F32 || isnan(x) || isinf(x) || x == 0.0f) return x;
+  const int bits = prec == GOR_STORE_BF16 ? 8 : 11;
+  const int qmin = prec == GOR_STORE_BF16 ? -133 : -24;
+  const double y = prec == GOR_STORE_BF16 ? (double)x : ldexp((double)x, shift);
+  int e;
+  (void)frexp(y, &e); /* |y| in [2^(e-1), 2^e) */
+  int qe = e - bits;
+  if (qe < qmin) qe = qmin;
+  double r = ldexp(nearbyint(ldexp(y, -qe)), qe);
+  if (prec == GOR_STORE_F16) {
+    if (fabs(r) > 65504.0) r = copysign(INFINITY, y);
+    return (float)ldexp(r, -shift);
+  }
+  return (float)r; /* above the largest bfloat16 value this is 2^128 -> inf */
+}
+
+void gor_round_trace_n(int prec, int shift, int64_t count, const float* in, float* out) {
+  for (int64_t i = 0; i < count; i++) out[i] = gor_round_trace(prec, in[i], shift);
+}
+
+int gor_set_trace_form(gor_net_t* n, int proj, int prec, int p_shift, int e_shift) {
+  if (proj < 0 || proj >= n->nproj || prec < GOR_STORE_F32 || prec > GOR_STORE_F16) return -1;
+  if (n->arith == GOR_ARITH_F64) return prec == GOR_STORE_F32 ? 0 : -1;
+  gor_proj_t* q = &n->projs[proj];
+  q->store_prec = prec;
+  q->p_shift = p_shift;
+  q->e_shift = e_shift;
+  const int64_t S = (int64_t)q->H * q->C * q->M;
+  float* pij = (float*)q->pij;
+  float* eij = (float*)q->eij;
+  for (int64_t i = 0; i < S; i++) {
+    pij[i] = gor_round_trace(prec, pij[i], p_shift);
+    eij[i] = gor_round_trace(prec, eij[i], e_shift);
+  }
+  return 0;
+}
+
 /* three instantiations of the Proj step */
 #define T float
 #define SFX(x) tab_##x
 #define ROW_CATCHUP(p, kp, pdt, a, b, c) tab_row(p, kp, pdt, a, b, c)
 #define SYN_CATCHUP(p, kp, pdt, a, b, c, d) tab_syn(p, kp, pdt, a, b, c, d)
 #define LOGT(x) gsbn_logf(x)
+#define STORE_P(p, x) gor_round_trace((p)->store_prec, (x), (p)->p_shift)
+#define STORE_E(p, x) gor_round_trace((p)->store_prec, (x), (p)->e_shift)
 #include "proj_step.inc"
 #undef T
 #undef SFX
 #undef ROW_CATCHUP
 #undef SYN_CATCHUP
 #undef LOGT
+#undef STORE_P
+#undef STORE_E
 
 #define T float
 #define SFX(x) r32_##x
 #define ROW_CATCHUP(p, kp, pdt, a, b, c) r32_row(p, kp, pdt, a, b, c)
 #define SYN_CATCHUP(p, kp, pdt, a, b, c, d) r32_syn(p, kp, pdt, a, b, c, d)
 #define LOGT(x) logf(x)
+#define STORE_P(p, x) gor_round_trace((p)->store_prec, (x), (p)->p_shift)
+#define STORE_E(p, x) gor_round_trace((p)->store_prec, (x), (p)->e_shift)
 #include "proj_step.inc"
 #undef T
 #undef SFX
 #undef ROW_CATCHUP
 #undef SYN_CATCHUP
 #undef LOGT
+#undef STORE_P
+#undef STORE_E
 
 #define T double
 #define SFX(x) f64_##x
 #define ROW_CATCHUP(p, kp, pdt, a, b, c) f64_row(p, kp, pdt, a, b, c)
 #define SYN_CATCHUP(p, kp, pdt, a, b, c, d) f64_syn(p, kp, pdt, a, b, c, d)
 #define LOGT(x) log(x)
+#define STORE_P(p, x) (x)
+#define STORE_E(p, x) (x)
 #include "proj_step.inc"
 #undef T
 #undef SFX
 #undef ROW_CATCHUP
 #undef SYN_CATCHUP
 #undef LOGT
+#undef STORE_P
+#undef STORE_E
 
 static void proj_step(gor_net_t* n, gor_proj_t* q) {
   if (n->arith == GOR_ARITH_TABLE)

@@ -77,6 +77,16 @@ void gor_exchange_spikes(gor_net_t* n);
  * are taken from sj (H*M int8) instead of a Pop update. */
 void gor_proj_step_replay(gor_net_t* n, int proj, const int8_t* sj);
 
+/* Storage form of Pij / Eij in the float variants: GOR_STORE_F32 (plain),
+ * GOR_STORE_BF16, GOR_STORE_F16 (binary16 of x * 2^shift); every stored
+ * value is rounded to nearest, ties to even, Wij uses the unrounded value.
+ * Rounds the current state; returns 0, or -1 for bad arguments or a
+ * 16-bit form on the double variant. */
+enum { GOR_STORE_F32 = 0, GOR_STORE_BF16 = 1, GOR_STORE_F16 = 2 };
+int gor_set_trace_form(gor_net_t* n, int proj, int prec, int p_shift, int e_shift);
+float gor_round_trace(int prec, float x, int shift);
+void gor_round_trace_n(int prec, int shift, int64_t count, const float* in, float* out);
+
 /* vector access by the reference's Database names, e.g. "pij_0", "spike_0".
  * Returns the element count and the element type code (0 i8, 2 i32, 5 f32,
  * 6 f64) or -1 if unknown; copies min(count, cap) elements when out != 0. */

@@ -87,12 +87,46 @@ def lib():
         L.gor_det_logf.argtypes = [f32]
         L.gor_det_expf.restype = f32
         L.gor_det_expf.argtypes = [f32]
+        L.gor_set_trace_form.restype = i32
+        L.gor_set_trace_form.argtypes = [vp, i32, i32, i32, i32]
+        L.gor_round_trace.restype = f32
+        L.gor_round_trace.argtypes = [i32, f32, i32]
+        L.gor_round_trace_n.argtypes = [i32, i32, i64, vp, vp]
         _lib = L
     return _lib
 
 
 def _p(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+TRACE_FORMS = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
+def round_trace(prec: int, x, shift: int = 0) -> np.ndarray:
+    """The oracle's 16-bit storage rounding applied to an array."""
+    a = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    o = np.empty_like(a)
+    lib().gor_round_trace_n(prec, shift, a.size, _p(a), _p(o))
+    return o.reshape(np.shape(x))
+
+
+def fp16_shift(q, dt: float) -> int:
+    """Scale exponent of the binary16 trace form for a projection: the
+    largest k with zmax_i * zmax_j * 2^k <= 65504, where a z trace kicked by
+    kft at most once per step and decaying by exp(-kz) per step never
+    exceeds kft / (1 - exp(-kz)).  Rates in float as the procedure computes
+    them (Proj.cpp:38-52)."""
+    f = np.float32
+    kzi = float(f(dt) / f(q.tauzi))
+    kzj = float(f(dt) / f(q.tauzj))
+    kfti = float(f(1) / (f(q.maxfq) * f(q.tauzi)))
+    kftj = float(f(1) / (f(q.maxfq) * f(q.tauzj)))
+    bound = max(1.0, kfti / -np.expm1(-kzi) * kftj / -np.expm1(-kzj))
+    k = int(np.floor(np.log2(65504.0 / bound)))
+    while bound * 2.0 ** k > 65504.0:
+        k -= 1
+    return k
 
 
 def philox(ctr, key) -> np.ndarray:
@@ -298,6 +332,14 @@ class OracleNet:
         self._mask = np.ascontiguousarray(spec.mask, dtype=np.float32)
         self._keep = []
         self._init_conn()
+        self.trace_shifts = []
+        prec = TRACE_FORMS[getattr(spec, "trace_precision", "fp32")]
+        if prec and arith != ARITH_F64:
+            for pid, q in enumerate(spec.projs):
+                k = fp16_shift(q, spec.dt) if prec == 2 else 0
+                self.trace_shifts.append(k)
+                if L.gor_set_trace_form(self.h, pid, prec, k, k) != 0:
+                    raise ValueError("oracle: bad trace form")
         if spec.ext_spikes:
             for pop, steps in spec.ext_spikes.items():
                 nsteps = max(steps) + 1 if steps else 0
