@@ -95,7 +95,9 @@ class NetSpec:
     check_threshold: int = 0
     ext_spikes: Optional[Dict[int, Dict[int, List[int]]]] = None  # pop -> step -> units
     name: str = "net"
-    exchange_first: bool = False  # list ProcExchangeSpike before ProcUpdLazy
+    # False: ProcExchangeSpike copies spikes into si_<proj> every step and the
+    # delay-queue push runs as its own kernel (argi direct-history 0)
+    exchange_direct: bool = True
     conn_text: Optional[str] = None  # verbatim connection-map text (overrides conn)
     # per projection (ii, di) arrays of shape [H*C], written through the
     # Database API instead of a connection-map file (no ProcInitConn)
@@ -241,8 +243,6 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
     L.append("}")
     L.append('rec_param { directory: "%s" }' % rec_dir)
     L.append('proc_param { name: "ProcExtGen" }')
-    if spec.exchange_first:
-        L.append('proc_param { name: "ProcExchangeSpike" }')
     L.append("proc_param {")
     L.append('  name: "ProcUpdLazy"')
     L.append('  argi { key: "seed-lo" val: %d }' % (spec.seed & 0x7FFFFFFF))
@@ -258,8 +258,10 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('  name: "ProcInitConn"')
         L.append('  args { key: "conn-map-file" val: "%s" }' % conn_path)
         L.append("}")
-    if not spec.exchange_first:
+    if spec.exchange_direct:
         L.append('proc_param { name: "ProcExchangeSpike" }')
+    else:
+        L.append('proc_param { name: "ProcExchangeSpike" argi { key: "direct-history" val: 0 } }')
     L.append("proc_param {")
     L.append('  name: "ProcCheck"')
     L.append('  argi { key: "threashold" val: %d }' % spec.check_threshold)

@@ -274,8 +274,10 @@ void ProcExchangeSpike::init_new(SolverParam solver_param, Database& db) {
   /* initialised after ProcUpdLazy: every executed step delivers the spikes
    * of that step to si after the lazy update, so the next step's queue push
    * sees them; ProcUpdLazy then takes the push from its own spike history */
-  int ready = 0;
-  if (_glv.geti("updlazy-ready", ready) && ready) {
+  int ready = 0, allow = 1;
+  Parser par(get_proc_param(solver_param));
+  par.argi("direct-history", allow); /* 0: keep the copy and the separate queue push */
+  if (allow && _glv.geti("updlazy-ready", ready) && ready) {
     _glv.puti("exchange-spike-direct", 1);
     _direct = true;
     /* si_<proj> is then only materialised when the host reads it: it equals

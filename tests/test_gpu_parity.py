@@ -62,10 +62,10 @@ def _lockstep_and_compare(spec, tmp_path, max_steps=None):
 
 
 def test_config0_queue_kernel_path(tmp_path):
-    # ProcExchangeSpike listed before ProcUpdLazy: the queue push runs as its
-    # own kernel from si_<proj>; same results as the direct path
+    # ProcExchangeSpike copies spikes into si_<proj> and the queue push runs
+    # as its own kernel (gsbn_proj_que); same results as the direct path
     spec = configs.config0()
-    spec.exchange_first = True
+    spec.exchange_direct = False
     gpu, ora = _lockstep_and_compare(spec, tmp_path, max_steps=600)
     gpu.close()
 
