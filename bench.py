@@ -17,6 +17,10 @@ roofline: the row/column kernel, timed with HIP events on the solver stream
 around every launch in the timed region; algorithmic bytes are 44 per
 synapse update (read pij, eij, zi2, zj2, tij; write them and wij).
 
+forms: the same workload with Pij/Eij stored in bf16 and in scaled fp16
+(36 algorithmic bytes per update), same steps, reported beside the plain
+FP32 value.
+
 cpu_baseline: this project's CPU restatement of the same procedure (the
 oracle, plain C, -O2, one thread pinned to one core) on the same workload:
 the same warm-up, then as many steps as fit in about 20 s.
@@ -39,7 +43,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("synapse-trace updates/sec + sim steps/sec at 100×100 HCU×MCU, plain vs "
           "BF16/batched")
-BYTES_PER_SYN_UPDATE = 44
+# algorithmic bytes per synapse update: FP32 form reads pij, eij, zi2, zj2, tij
+# and writes them and wij (row) or reads/writes the five and reads ii
+# (column); 16-bit forms move 2-byte pij/eij
+BYTES_PER_SYN_UPDATE = {"fp32": 44, "bf16": 36, "fp16": 36}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -97,28 +104,10 @@ def cpu_only(args):
                                 "1 thread pinned to one core" % (n, args.warmup, dt)}))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="100x100")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-only", action="store_true")
-    ap.add_argument("--step-times", type=int, default=0,
-                    help="after the timed region, time this many synchronised single steps")
-    args = ap.parse_args()
-    if args.cpu_only:
-        return cpu_only(args)
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU form is a design only "
-              "(see DESIGN.md). Use --gpus 1.", file=sys.stderr)
-        sys.exit(2)
-
+def measure(spec, steps, warmup, step_times=0):
+    """Build the solver, run `warmup` steps, time `steps` steps; HIP events
+    around every row/column launch of the timed region."""
     from gsbn_amd import Solver, configs
-
-    spec, desc = workload(args.workload)
     with tempfile.TemporaryDirectory() as d:
         path = configs.write(spec, d)
         t_init = time.perf_counter()
@@ -126,22 +115,22 @@ def main():
         configs.apply_conn_arrays(s, spec)
         s.sync()
         t_init = time.perf_counter() - t_init
-        s.step(args.warmup)
+        s.step(warmup)
         s.sync()
         s.timing(-1, collect=False)
         w0 = s.work()
         s.timing(1 << 1, collect=False)  # HIP events around every row/column launch
         t0 = time.perf_counter()
-        done = s.step(args.steps)
+        done = s.step(steps)
         s.sync()
         t1 = time.perf_counter()
         launches, ms = s.timing(0)
         w1 = s.work()
         step_ms = None
-        if args.step_times:
+        if step_times:
             # per-step wall times (each step synchronised) for the p50/p99 report
             ts = []
-            for _ in range(args.step_times):
+            for _ in range(step_times):
                 a0 = time.perf_counter()
                 s.step(1)
                 s.sync()
@@ -153,25 +142,55 @@ def main():
     rows = int(w1[0] - w0[0])
     cols = int(w1[1] - w0[1])
     sweep = int(w1[2] - w0[2])
-    updates = rows + cols + sweep
-    value = updates / el
+    bpu = BYTES_PER_SYN_UPDATE[spec.trace_precision]
     rc_launch, rc_ms = int(launches[1]), float(ms[1])
     avg_ms = rc_ms / max(rc_launch, 1)
-    bytes_per_launch = BYTES_PER_SYN_UPDATE * (rows + cols) / max(rc_launch, 1)
+    bytes_per_launch = bpu * (rows + cols) / max(rc_launch, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"elapsed": el, "done": int(done), "rows": rows, "cols": cols, "sweep": sweep,
+            "updates_per_s": (rows + cols + sweep) / el, "steps_per_s": done / el,
+            "ms_per_step": 1e3 * el / max(done, 1), "init_s": t_init, "step_ms": step_ms,
+            "rc_launches": rc_launch, "rc_avg_ms": avg_ms, "rc_bytes_per_launch": bytes_per_launch,
+            "rc_achieved_gbs": achieved}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="100x100")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the bf16 / fp16 trace-storage runs of the same workload")
+    ap.add_argument("--cpu-only", action="store_true")
+    ap.add_argument("--step-times", type=int, default=0,
+                    help="after the timed region, time this many synchronised single steps")
+    args = ap.parse_args()
+    if args.cpu_only:
+        return cpu_only(args)
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU form is a design only "
+              "(see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    spec, desc = workload(args.workload)
+    m = measure(spec, args.steps, args.warmup, args.step_times)
     traffic = None
-    tp = os.path.join(ROOT, "profiles", "rowcol_100x100_traffic.json")
+    tp = os.path.join(ROOT, "profiles", "rowcol_%s_traffic.json" % args.workload)
     if os.path.exists(tp):
         with open(tp) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    achieved = m["rc_achieved_gbs"]
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": m["updates_per_s"],
         "unit": "synapse-trace updates/s",
         "n_gpus": 1,
-        "steps": int(done),
+        "steps": m["done"],
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / max(done, 1),
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -181,20 +200,33 @@ def main():
                    "hcu": spec.total_hcu, "units": spec.total_units,
                    "synapses": sum(p.hcu * int(q.slot_num) * p.mcu
                                    for p, q in zip(spec.pops, spec.projs)),
-                   "phase": "training steps %d..%d" % (args.warmup + 1, args.warmup + done)},
-        "steps_per_sec": done / el,
-        "work": {"row_synapse_updates": rows, "col_synapse_updates": cols,
-                 "sweep_synapse_updates": sweep},
-        "init_s": t_init,
+                   "phase": "training steps %d..%d" % (args.warmup + 1, args.warmup + m["done"])},
+        "steps_per_sec": m["steps_per_s"],
+        "work": {"row_synapse_updates": m["rows"], "col_synapse_updates": m["cols"],
+                 "sweep_synapse_updates": m["sweep"]},
+        "init_s": m["init_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "rowcol_kernel", "avg_launch_ms": avg_ms,
-                     "launches": rc_launch, "bytes_per_launch": bytes_per_launch},
+                     "kernel": "rowcol_kernel", "avg_launch_ms": m["rc_avg_ms"],
+                     "launches": m["rc_launches"], "bytes_per_launch": m["rc_bytes_per_launch"]},
     }
-    if step_ms is not None:
-        line["step_ms"] = step_ms
+    if m["step_ms"] is not None:
+        line["step_ms"] = m["step_ms"]
     if len(spec.pops) > 1:
-        line["network_steps_per_sec"] = len(spec.pops) * done / el
+        line["network_steps_per_sec"] = len(spec.pops) * m["done"] / m["elapsed"]
+    if not args.no_forms and args.workload == "100x100":
+        forms = {}
+        for form in ("bf16", "fp16"):
+            fspec, _ = workload(args.workload)
+            fspec.trace_precision = form
+            f = measure(fspec, args.steps, args.warmup)
+            forms[form] = {"value": f["updates_per_s"], "steps_per_sec": f["steps_per_s"],
+                           "ms_per_step": f["ms_per_step"],
+                           "speedup_steps_per_sec": f["steps_per_s"] / m["steps_per_s"],
+                           "rowcol_avg_launch_ms": f["rc_avg_ms"],
+                           "rowcol_achieved_gbs": f["rc_achieved_gbs"],
+                           "bytes_per_update": BYTES_PER_SYN_UPDATE[form]}
+        line["forms"] = forms
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line))

@@ -174,6 +174,7 @@ void ProcInitConn::init_new(SolverParam solver_param, Database& db) {
     int dim_conn = 0, dim_hcu = 0, nsrc = 0;
     SyncVector<int32_t>* ii = nullptr;
     SyncVector<int32_t>* di = nullptr;
+    vector<int> cursor; /* per destination HCU: no free slot below this index */
   };
   vector<std::pair<int, int>> pops;  // hcu, mcu
   for (auto& pp : solver_param.net_param.pop_param)
@@ -216,18 +217,21 @@ void ProcInitConn::init_new(SolverParam solver_param, Database& db) {
     if ((v.size() - 2) % 2) throw Error(-2, "conn-map: unpaired source/delay");
     int* pii = p.ii->mutable_cpu_data() + (int64_t)hcu_id * p.dim_conn;
     int* pdi = p.di->mutable_cpu_data() + (int64_t)hcu_id * p.dim_conn;
+    /* ProcInitConn::add_row: first free slot (ii < 0) of the hypercolumn,
+     * or nothing when all are taken.  Slots only go from free to taken
+     * here, so the first free slot never moves backwards: a per-HCU cursor
+     * finds it without rescanning the taken prefix. */
+    if (p.cursor.empty()) p.cursor.assign(p.dim_hcu, 0);
+    int& k = p.cursor[hcu_id];
     for (size_t i = 2; i + 1 < v.size(); i += 2) {
       int ii = std::stoi(v[i]);
       int di = std::stoi(v[i + 1]);
       if (ii < 0 || ii >= p.nsrc) throw Error(-2, "conn-map: source unit out of range");
       if (di < 1 || di > 31) throw Error(-2, "conn-map: delay must be in [1, 31]");
-      /* ProcInitConn::add_row: first free slot of the hypercolumn */
-      for (int k = 0; k < p.dim_conn; k++) {
-        if (pii[k] < 0) {
-          pii[k] = ii;
-          pdi[k] = di;
-          break;
-        }
+      while (k < p.dim_conn && pii[k] >= 0) ++k;
+      if (k < p.dim_conn) {
+        pii[k] = ii;
+        pdi[k] = di;
       }
     }
   }

@@ -16,12 +16,14 @@
  *                            REF32/TABLE/F64 variants against each other
  *                            (test_oracle_step.py)
  */
+#define _DEFAULT_SOURCE /* posix_memalign, madvise */
 #include "gsbn_oracle.h"
 
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
 
 #include "detmath.h"
 
@@ -288,8 +290,19 @@ struct gor_net {
 
 static size_t tsize(const gor_net_t* n) { return n->arith == GOR_ARITH_F64 ? sizeof(double) : sizeof(float); }
 
+/* Large arrays are 2 MiB aligned and marked for transparent huge pages:
+ * the column updates walk five per-synapse arrays at a stride of M floats
+ * (the reference's layout), which with 4 KiB pages misses the TLB on almost
+ * every access. */
 static void* zalloc(size_t count, size_t sz) {
-  void* p = calloc(count ? count : 1, sz);
+  const size_t bytes = (count ? count : 1) * sz;
+  void* p = NULL;
+  if (bytes >= ((size_t)4 << 20) && posix_memalign(&p, (size_t)2 << 20, bytes) == 0) {
+    (void)madvise(p, bytes, MADV_HUGEPAGE);
+    memset(p, 0, bytes);
+    return p;
+  }
+  p = calloc(count ? count : 1, sz);
   if (!p) {
     fprintf(stderr, "oracle: out of memory\n");
     abort();

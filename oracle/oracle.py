@@ -368,14 +368,19 @@ class OracleNet:
             C = sp.hcu * sp.mcu
             if C > q.slot_num:  # Proj.cpp:22-26
                 C = int(q.slot_num)
+            # first free slot of the HCU, or none: slots only go from free to
+            # taken, so a per-HCU cursor never has to look behind itself
+            cursor = {}
             for dest_hcu, pairs in rows:
                 base = dest_hcu * C
+                k = cursor.get(dest_hcu, 0)
                 for s, d in pairs:
-                    for k in range(C):
-                        if ii[base + k] < 0:
-                            ii[base + k] = s
-                            di[base + k] = d
-                            break
+                    while k < C and ii[base + k] >= 0:
+                        k += 1
+                    if k < C:
+                        ii[base + k] = s
+                        di[base + k] = d
+                cursor[dest_hcu] = k
             self.write("ii_%d" % pid, ii)
             self.write("di_%d" % pid, di)
             del dp

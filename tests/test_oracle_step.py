@@ -224,3 +224,33 @@ def test_extgen_time_truncation_quirk():
     assert int(np.float32(0.005) / np.float32(0.001)) == 4
     for k in range(0, 20000, 7):
         assert int(np.float32(configs.step_time(k, 0.001)) / np.float32(0.001)) == k
+
+
+def test_conn_slot_fill_matches_first_free_scan():
+    # the oracle's cursor fill against a literal restatement of
+    # ProcInitConn::add_row (scan for the first ii < 0, drop when full), on a
+    # map with repeated destination lines and more sources than slots
+    rng = np.random.default_rng(9)
+    H, M, C = 4, 3, 7
+    rows = []
+    for _ in range(10):
+        h = int(rng.integers(0, H))
+        pairs = [(int(rng.integers(0, H * M)), int(rng.integers(1, 4)))
+                 for _ in range(int(rng.integers(0, 5)))]
+        rows.append((h, pairs))
+    spec = tiny_spec(H=H, M=M, steps=3)
+    spec.projs[0].slot_num = float(C)
+    spec.conn = [rows]
+    net = O.OracleNet(spec)
+    want_ii = np.full(H * C, -1, np.int32)
+    want_di = np.zeros(H * C, np.int32)
+    for h, pairs in rows:
+        for s, d in pairs:
+            for k in range(C):
+                if want_ii[h * C + k] < 0:
+                    want_ii[h * C + k] = s
+                    want_di[h * C + k] = d
+                    break
+    assert (net.vector("ii_0") == want_ii).all()
+    assert (net.vector("di_0") == want_di).all()
+    assert (want_ii >= 0).sum() < sum(len(p) for _, p in rows)  # some sources were dropped
