@@ -42,6 +42,14 @@ int guard(F f) {
   }
 }
 
+}  // namespace
+
+namespace gsbn {
+/* message for gsbn_last_error() from the other C ABI sources */
+void set_last_error(const std::string& m) { g_err = m; }
+}  // namespace gsbn
+
+namespace {
 bool dynamic_size(const std::string& n) {
   return n.rfind(".ssi_", 0) == 0 || n.rfind(".ssj_", 0) == 0;
 }

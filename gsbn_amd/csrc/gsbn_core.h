@@ -63,17 +63,40 @@ enum {
  * update took the D<=0 branch (Proj.cpp:428-431 adds nothing to epsc) */
 #define GSBN_NO_CONTRIB_BITS 0x7fbadbadu
 
-/* Synapse catch-up over a gap with coefficients c (pointer to one table
- * entry).  Row form adds kfti to zi2, column form adds kftj to zj2. */
-GSBN_HD GSBN_INLINE void gsbn_syn_catchup(const float* c, float* pij, float* eij,
-                                          float* zi2, float* zj2) {
+/* the seven coefficients a synapse update uses */
+typedef struct {
+  float ezi, ezj, ee, ep, cez, cpe, cpz;
+} gsbn_syn_coef_t;
+
+GSBN_HD GSBN_INLINE gsbn_syn_coef_t gsbn_syn_coef(const float* c) {
+  gsbn_syn_coef_t k;
+  k.ezi = c[GSBN_C_EZI];
+  k.ezj = c[GSBN_C_EZJ];
+  k.ee = c[GSBN_C_EE];
+  k.ep = c[GSBN_C_EP];
+  k.cez = c[GSBN_C_EZ_IJ];
+  k.cpe = c[GSBN_C_PE];
+  k.cpz = c[GSBN_C_PZ_IJ];
+  return k;
+}
+
+/* Synapse catch-up over a gap.  Row form adds kfti to zi2, column form adds
+ * kftj to zj2 (by the caller). */
+GSBN_HD GSBN_INLINE void gsbn_syn_catchup_k(const gsbn_syn_coef_t k, float* pij, float* eij,
+                                            float* zi2, float* zj2) {
   float z0 = (*zi2) * (*zj2);
-  float p = (c[GSBN_C_EP] * (*pij) + c[GSBN_C_PE] * (*eij)) + c[GSBN_C_PZ_IJ] * z0;
-  float e = c[GSBN_C_EE] * (*eij) + c[GSBN_C_EZ_IJ] * z0;
+  float p = (k.ep * (*pij) + k.cpe * (*eij)) + k.cpz * z0;
+  float e = k.ee * (*eij) + k.cez * z0;
   *pij = p;
   *eij = e;
-  *zi2 = c[GSBN_C_EZI] * (*zi2);
-  *zj2 = c[GSBN_C_EZJ] * (*zj2);
+  *zi2 = k.ezi * (*zi2);
+  *zj2 = k.ezj * (*zj2);
+}
+
+/* the same with coefficients c (pointer to one table entry) */
+GSBN_HD GSBN_INLINE void gsbn_syn_catchup(const float* c, float* pij, float* eij,
+                                          float* zi2, float* zj2) {
+  gsbn_syn_catchup_k(gsbn_syn_coef(c), pij, eij, zi2, zj2);
 }
 
 GSBN_HD GSBN_INLINE void gsbn_row_catchup(const float* c, float* pi, float* ei, float* zi) {

@@ -35,6 +35,7 @@ static_assert(sizeof(gsbn_syn16_rec) == 16 && sizeof(gsbn_syn16_t) == 16,
 
 #define POP_MAX_THREADS 1024
 #define RC_THREADS 256
+#define RC_COLS 4 /* column slots per thread and pass */
 #define ALL_THREADS 256
 #define ALL_ROWS 16
 
@@ -89,6 +90,20 @@ __device__ float finalized_epsc(const gsbn_pop_args& a, const gsbn_pop_proj& p, 
   return e;
 }
 
+/* queue push of source unit n spiking at exec step s: its rows become
+ * active at s + di + 2; threads tid, tid+nt, ... take the unit's rows */
+__device__ __forceinline__ void push_unit(const gsbn_fanout& f, int n, int s, int tid, int nt) {
+  const int q1 = f.fwd_off[n + 1];
+  for (int q = f.fwd_off[n] + tid; q < q1; q += nt) {
+    const int r = f.fwd_row[q];
+    const int d = f.fwd_del[q];
+    const int hh = r / f.C;
+    const int k = r - hh * f.C;
+    const int slot = (s + d + 2) % GSBN_FUT;
+    atomicOr(f.fut + ((int64_t)slot * f.H + hh) * f.W + (k >> 5), 1u << (k & 31));
+  }
+}
+
 /* support integration of one unit (update_sup_kernel_1_cpu, Pop.cpp:229-261);
  * returns the new dsup and the draws */
 __device__ float unit_dsup(const gsbn_pop_args& a, int h, int j, const float* efin, float* u,
@@ -119,6 +134,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   extern __shared__ uint32_t sh_bitmap[];
   __shared__ float sh_f[POP_MAX_THREADS];
   __shared__ int sh_ext[POP_MAX_THREADS];
+  __shared__ int sh_spk[POP_MAX_THREADS];
   __shared__ int sh_scan[POP_MAX_THREADS / 64 + 1];
   __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
   __shared__ float sh_vsum;
@@ -150,18 +166,26 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   }
   __syncthreads();
 
-  /* 3. WTA maximum (update_sup_kernel_2_cpu, Pop.cpp:270-279); tree over
-   * the next power of two >= M, reading only indices < M */
-  int p2 = 1;
-  while (p2 < M) p2 <<= 1;
-  for (int s = p2 / 2; s > 0; s >>= 1) {
-    if (j < s && j + s < M) {
-      float o = sh_f[j + s];
-      if (o > sh_f[j]) sh_f[j] = o;
+  /* 3. WTA maximum (update_sup_kernel_2_cpu, Pop.cpp:270-279): the
+   * reference's sequential `if (d > max) max = d` keeps the first of equal
+   * maxima; combining contiguous index ranges left to right with the same
+   * comparison gives the same value (also for +-0 ties and NaN), here
+   * within each wave by shuffles, then across waves in order */
+  {
+    float mv = live ? dsup : -INFINITY;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const float o = __shfl_down(mv, off, 64);
+      if ((j & (2 * off - 1)) == 0 && o > mv) mv = o;
     }
-    __syncthreads();
+    if ((j & 63) == 0) sh_f[j >> 6] = mv;
   }
+  __syncthreads();
   float maxdsup = sh_f[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+    const float o = sh_f[w];
+    if (o > maxdsup) maxdsup = o;
+  }
   if (a.wta_pop_max) {
     float d0 = a.scratch_dsup0[0];
     float mx = d0;
@@ -214,13 +238,19 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   int spk_tot;
   {
     int pos = block_excl_scan(live ? spk : 0, sh_scan, &spk_tot);
-    if (live && spk) a.spk_list[h * M + pos] = j;
-    if (j == 0) a.spk_cnt[h] = spk_tot;
-    if (a.hist_out_list) {
-      const int slot = (a.exec + 1) % GSBN_HIST;
-      if (live && spk) a.hist_out_list[((int64_t)slot * a.H + h) * M + pos] = j;
-      if (j == 0) a.hist_out_cnt[(int64_t)slot * a.H + h] = spk_tot;
+    if (live && spk) {
+      a.spk_list[h * M + pos] = j;
+      sh_spk[pos] = j;
     }
+    if (j == 0) a.spk_cnt[h] = spk_tot;
+  }
+  /* 5b. queue push of this step's spikes (what the queue update of the next
+   * step would push from si, Proj.cpp:645-664) into the outgoing rings */
+  if (a.nproj_out > 0 && spk_tot > 0) {
+    __syncthreads();
+    for (int o = 0; o < a.nproj_out; ++o)
+      for (int i = 0; i < spk_tot; ++i)
+        push_unit(a.outs[o], h * M + sh_spk[i], a.exec, threadIdx.x, blockDim.x);
   }
 
   /* 6. postsynaptic traces and bias for every incoming projection
@@ -242,32 +272,17 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
     }
   }
 
-  /* 7. active rows of this step from the delay queue (Proj.cpp:621-630,
-   * 645-664): row r = h*C+k is active at exec x iff its source unit ii was
-   * pushed at the queue update of exec x - di - 1 */
+  /* 7. active rows of this step (Proj.cpp:621-630): the ring slot of this
+   * exec step, read into LDS and cleared for its next use */
   for (int m = 0; m < a.nproj_in; ++m) {
     const gsbn_pop_proj& p = a.projs[m];
     const int C = p.C;
-    const int W = (C + 31) >> 5;
-    for (int w = j; w < W; w += blockDim.x) sh_bitmap[w] = 0u;
-    __syncthreads();
-    const int32_t* off = p.inv_off + (int64_t)h * (p.Nsrc + 1);
-    for (int d = p.dmin; d <= p.dmax; ++d) {
-      const int e = a.exec - d - 1;
-      if (e < 0) continue;
-      const int slot = e % GSBN_HIST;
-      const int32_t* cnts = p.hist_cnt + (int64_t)slot * p.Hsrc;
-      const int32_t* lsts = p.hist_list + (int64_t)slot * p.Hsrc * p.Msrc;
-      for (int hs = j; hs < p.Hsrc; hs += blockDim.x) {
-        const int cnt = cnts[hs];
-        for (int t = 0; t < cnt; ++t) {
-          const int n = hs * p.Msrc + lsts[(int64_t)hs * p.Msrc + t];
-          for (int q = off[n]; q < off[n + 1]; ++q) {
-            const int k = p.inv_slot[q];
-            if (p.di[(int64_t)h * C + k] == d) atomicOr(&sh_bitmap[k >> 5], 1u << (k & 31));
-          }
-        }
-      }
+    const int W = p.W;
+    uint32_t* words = p.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * W;
+    for (int w = j; w < W; w += blockDim.x) {
+      const uint32_t v = words[w];
+      sh_bitmap[w] = v;
+      if (v) words[w] = 0u;
     }
     __syncthreads();
     const int per = (W + blockDim.x - 1) / blockDim.x;
@@ -349,6 +364,23 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_arg
   }
   __syncthreads();
   if (j < a.nproj_in) a.projs[j].pend_cnt[h] = 0;
+}
+
+/* the seven synapse coefficients of gap D: two 16-byte loads (entries are
+ * 48 bytes, 16-byte aligned) */
+__device__ __forceinline__ gsbn_syn_coef_t load_coef(const float* table, int len, int D) {
+  const float* c = gsbn_coef(table, len, D);
+  const float4 x = *reinterpret_cast<const float4*>(c);
+  const float4 y = *reinterpret_cast<const float4*>(c + 4);
+  gsbn_syn_coef_t k;
+  k.ezi = x.x;
+  k.ezj = x.y;
+  k.ee = x.z;
+  k.ep = x.w;
+  k.cez = y.x;
+  k.cpe = y.y;
+  k.cpz = y.z;
+  return k;
 }
 
 /* ---- synapse state access for the FP32 record and the 16-bit forms ---- */
@@ -499,8 +531,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
         s.tij = t;
         contrib = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
       } else {
-        const float* c = gsbn_coef(q.table, q.table_len, D);
-        gsbn_syn_catchup(c, &s.pij, &s.eij, &s.zi2, &s.zj2);
+        gsbn_syn_catchup_k(load_coef(q.table, q.table_len, D), &s.pij, &s.eij, &s.zi2, &s.zj2);
         s.zi2 = s.zi2 + a.kfti;
         s.tij = t;
         if (learn) {
@@ -524,23 +555,53 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   const int slice = (C + P - 1) / P;
   const int k0 = part * slice;
   const int k1 = (k0 + slice < C) ? k0 + slice : C;
+  /* RC_COLS slots per thread and pass, in phases so that the loads of
+   * different slots are in flight together: slot checks, records,
+   * coefficients, then the updates */
+  constexpr int U = RC_COLS;
   for (int s_i = 0; s_i < nspk; ++s_i) {
     const int j = a.spk_list[h * M + s_i];
-    for (int k = k0 + tid; k < k1; k += RC_THREADS) {
-      const int r = h * C + k;
-      if (a.ii[r] < 0 || a.act_stamp[r] == q.exec) continue;
-      const int64_t sidx = (int64_t)r * M + j;
-      SynVals v = io.load_traces(sidx);
-      const int D = t - v.tij;
-      if (D <= 0) {
-        v.zj2 += a.kftj;
-      } else {
-        const float* c = gsbn_coef(q.table, q.table_len, D);
-        gsbn_syn_catchup(c, &v.pij, &v.eij, &v.zi2, &v.zj2);
-        v.zj2 = v.zj2 + a.kftj;
+    for (int kb = k0 + tid; kb < k1; kb += U * RC_THREADS) {
+      bool ok[U];
+      int64_t sidx[U];
+      int iiv[U], stamp[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = kb + u * RC_THREADS;
+        const int r = h * C + (k < k1 ? k : k0);
+        sidx[u] = (int64_t)r * M + j;
+        iiv[u] = a.ii[r];
+        stamp[u] = a.act_stamp[r];
+        ok[u] = k < k1;
       }
-      v.tij = t;
-      io.store_traces(sidx, v);
+      SynVals v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ok[u] = ok[u] & (iiv[u] >= 0) & (stamp[u] != q.exec);
+        if (ok[u])
+          v[u] = io.load_traces(sidx[u]);
+        else
+          v[u].tij = t;
+      }
+      gsbn_syn_coef_t cf[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int D = t - v[u].tij;
+        if (ok[u] && D > 0) cf[u] = load_coef(q.table, q.table_len, D);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        const int D = t - v[u].tij;
+        if (D <= 0) {
+          v[u].zj2 += a.kftj;
+        } else {
+          gsbn_syn_catchup_k(cf[u], &v[u].pij, &v[u].eij, &v[u].zi2, &v[u].zj2);
+          v[u].zj2 = v[u].zj2 + a.kftj;
+        }
+        v[u].tij = t;
+        io.store_traces(sidx[u], v[u]);
+      }
     }
   }
 }
@@ -599,8 +660,7 @@ __global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
       if (D <= 0) {
         io.store_tij(sidx, t);
       } else {
-        const float* c = gsbn_coef(a.table, a.table_len, D);
-        gsbn_syn_catchup(c, &s.pij, &s.eij, &s.zi2, &s.zj2);
+        gsbn_syn_catchup_k(load_coef(a.table, a.table_len, D), &s.pij, &s.eij, &s.zi2, &s.zj2);
         s.tij = t;
         if (learn) {
           const int h = (int)(r / a.C);
@@ -631,20 +691,11 @@ __global__ void jxx_kernel(gsbn_jxx_args a) {
 
 /* one workgroup per source HCU; ascending MCU order within the HCU */
 __global__ void __launch_bounds__(1024) que_kernel(gsbn_que_args a) {
-  __shared__ int sh_scan[1024 / 64 + 1];
   const int hs = blockIdx.x;
-  const int slot = a.exec % GSBN_HIST;
-  int32_t* out = a.hist_list + ((int64_t)slot * a.Hsrc + hs) * a.Msrc;
-  int running = 0;
-  for (int base = 0; base < a.Msrc; base += blockDim.x) {
-    const int j = base + threadIdx.x;
-    const int flag = (j < a.Msrc && a.si[hs * a.Msrc + j] > 0) ? 1 : 0;
-    int tot;
-    const int pos = block_excl_scan(flag, sh_scan, &tot);
-    if (flag) out[running + pos] = j;
-    running += tot;
+  for (int j = 0; j < a.Msrc; ++j) {
+    const int n = hs * a.Msrc + j;
+    if (a.si[n] > 0) push_unit(a.out, n, a.exec - 1, threadIdx.x, blockDim.x);
   }
-  if (threadIdx.x == 0) a.hist_cnt[(int64_t)slot * a.Hsrc + hs] = running;
 }
 
 __global__ void check_count_kernel(int n, const int8_t* spike, int32_t* count) {
@@ -834,7 +885,8 @@ extern "C" {
 
 int gsbn_pop_update(const gsbn_pop_args* a, void* stream) {
   if (!a || a->H <= 0 || a->M <= 0 || a->M > POP_MAX_THREADS || a->nproj_in < 0 ||
-      a->nproj_in > GSBN_MAX_PROJ_IN || (a->nproj_in > 0 && !a->projs) || a->max_slots < 0)
+      a->nproj_in > GSBN_MAX_PROJ_IN || (a->nproj_in > 0 && !a->projs) || a->max_slots < 0 ||
+      a->nproj_out < 0 || (a->nproj_out > 0 && !a->outs))
     return GSBN_ERR_ARG;
   if (a->wta_pop_max && !a->scratch_dsup0) return GSBN_ERR_ARG;
   const size_t lds = (size_t)((a->max_slots + 31) / 32) * sizeof(uint32_t);
@@ -964,7 +1016,10 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
 }
 
 int gsbn_proj_que(const gsbn_que_args* a, void* stream) {
-  if (!a || a->Hsrc <= 0 || a->Msrc <= 0 || a->exec < 0) return GSBN_ERR_ARG;
+  if (!a || a->Hsrc <= 0 || a->Msrc <= 0 || a->exec < 0 || !a->si || !a->out.fwd_off ||
+      !a->out.fwd_row || !a->out.fwd_del || !a->out.fut || a->out.H <= 0 || a->out.C <= 0 ||
+      a->out.W != (a->out.C + 31) / 32)
+    return GSBN_ERR_ARG;
   const int threads = a->Msrc >= 1024 ? 1024 : ((a->Msrc + 63) / 64) * 64;
   hipLaunchKernelGGL(que_kernel, dim3(a->Hsrc), dim3(threads), 0, (hipStream_t)stream, *a);
   return launch_status();

@@ -8,9 +8,10 @@
  * Proj.cpp:64-86).  The per-synapse and per-row traces live in record arrays
  * on the device; their named vectors (pij_<id>, ..., pi_<id>, ...) are views
  * that gather a field on host reads and scatter it back before the next step
- * after host writes.  The delay queue is kept as a ring of the source units
- * pushed at each executed step; qi_<id>, .siq_<id> and .ssi_<id> are
- * materialised from it on host reads.
+ * after host writes.  The delay queue is a ring of per-step bitmaps of
+ * pending row activations (a spike pushes its rows into the slot of the step
+ * they become active); qi_<id>, .siq_<id> and .ssi_<id> are materialised from
+ * it on host reads.
  *
  * Differences from the reference, by design: update_cpu() raises (this
  * build's step runs on the GPU only; there is no silent CPU fallback),
@@ -83,8 +84,10 @@ class Pop {
   SyncVector<int32_t>* _counter = nullptr;
   vector<Proj*> _in; /* incoming projections in proj_in_pop order */
   DevBuf<int32_t> spk_list, spk_cnt, ext_off, ext_units;
-  DevBuf<int32_t> hist_list, hist_cnt; /* spike-history ring, queue format */
-  bool write_hist = false;
+  vector<Proj*> _out;       /* projections with this population as source */
+  bool push_direct = false; /* this population's kernel does their queue push */
+  DevBuf<gsbn_fanout> outs_dev;
+  vector<gsbn_fanout> outs_host;
   DevBuf<float> scratch;
   DevBuf<gsbn_pop_proj> projs_dev;
   DevBuf<int64_t> work;
@@ -187,13 +190,28 @@ class Proj {
   }
   DevBuf<gsbn_row_rec> rowst;
   DevBuf<float> contrib;
-  DevBuf<int32_t> rows_list, rows_cnt, pend_cnt, act_stamp, hist_list, hist_cnt, inv_off, inv_slot;
+  DevBuf<int32_t> rows_list, rows_cnt, pend_cnt, act_stamp;
+  /* delay queue: rows of each source unit (CSR) with their delays, and the
+   * ring of pending activations, GSBN_FUT slots x H x W words */
+  DevBuf<int32_t> fwd_off, fwd_row;
+  DevBuf<uint8_t> fwd_del;
+  DevBuf<uint32_t> fut;
+  int W = 0;
+  gsbn_fanout fanout() const {
+    gsbn_fanout f;
+    memset(&f, 0, sizeof(f));
+    f.fwd_off = fwd_off.p;
+    f.fwd_row = fwd_row.p;
+    f.fwd_del = fwd_del.p;
+    f.fut = fut.p;
+    f.H = _dim_hcu;
+    f.C = _dim_conn;
+    f.W = W;
+    return f;
+  }
   uint64_t ii_version = ~0ull, di_version = ~0ull;
-  int dmin = 1, dmax = 0;
   int last_exec = -1;
-  bool direct_que = false; /* queue pushes read from the source population's history */
-  const int32_t* ring_list() const { return direct_que ? _ptr_src_pop->hist_list.p : hist_list.p; }
-  const int32_t* ring_cnt() const { return direct_que ? _ptr_src_pop->hist_cnt.p : hist_cnt.p; }
+  bool direct_que = false; /* the source population's kernel pushes this step's spikes */
   int sim_end = 0;
   std::vector<std::unique_ptr<VecView>> view_objs;
 };
@@ -295,52 +313,41 @@ void QueueView::pull(VecBase* v) {
     static_cast<SyncVector<int32_t>*>(v)->set_logical_size(out.size());
     return;
   }
-  /* qi / siq from the ring: que at exec e pushed bit (di-1); after the que
-   * of exec x it sits at bit (di-1)-(x-e); siq(x) = pushed at x-di */
+  /* qi / siq from the ring of pending activations.  After the queue update
+   * of exec step x, bit b of the reference's qi[r] means that row r becomes
+   * active at exec x + 2 + b (b < di[r]; a bit at b = di[r] would be a spike
+   * of step x itself, which the reference pushes only at step x + 1), and
+   * siq[r] that it is active at x + 1. */
   const int x = p->last_exec;
-  const int Hs = p->_ptr_src_pop->_dim_hcu, Ms = p->_ptr_src_pop->_dim_mcu;
-  vector<int32_t> hc((size_t)GSBN_HIST * Hs), hl((size_t)GSBN_HIST * Hs * Ms);
+  const int W = p->W;
+  vector<uint32_t> ring((size_t)GSBN_FUT * H * W);
   hip_check(hipStreamSynchronize(active_stream()), "sync");
-  hip_check(hipMemcpy(hc.data(), p->ring_cnt(), hc.size() * 4, hipMemcpyDeviceToHost), "D2H");
-  hip_check(hipMemcpy(hl.data(), p->ring_list(), hl.size() * 4, hipMemcpyDeviceToHost), "D2H");
-  const int lo = std::max(0, x - 32);
-  const int span = x - lo + 1;
-  vector<uint8_t> present;
-  if (x >= 0) {
-    present.assign((size_t)span * p->_nsrc, 0);
-    for (int e = lo; e <= x; ++e) {
-      const int slot = e % GSBN_HIST;
-      for (int hs = 0; hs < Hs; ++hs) {
-        const int c = hc[(size_t)slot * Hs + hs];
-        for (int i = 0; i < c; ++i)
-          present[(size_t)(e - lo) * p->_nsrc + hs * Ms + hl[((size_t)slot * Hs + hs) * Ms + i]] = 1;
-      }
-    }
-  }
+  if (!ring.empty())
+    hip_check(hipMemcpy(ring.data(), p->fut.p, ring.size() * 4, hipMemcpyDeviceToHost), "D2H");
+  auto pending = [&](int step, int64_t r) -> bool {
+    const int h = (int)(r / C), k = (int)(r - (int64_t)h * C);
+    const uint32_t w = ring[((size_t)(step % GSBN_FUT) * H + h) * W + (k >> 5)];
+    return ((w >> (k & 31)) & 1u) != 0;
+  };
   const int* ii = p->_ii->cpu_data();
   const int* di = p->_di->cpu_data();
-  auto in_hist = [&](int e, int n) -> bool {
-    if (e < lo || e > x || e < 0) return false;
-    return present[(size_t)(e - lo) * p->_nsrc + n] != 0;
-  };
   if (what == 0) {
     auto& out = raw<int32_t>(v);
     out.assign((size_t)R, 0);
-    for (int64_t r = 0; r < R; ++r) {
-      if (ii[r] < 0 || x < 0) continue;
-      const int d = di[r];
-      int q = 0;
-      for (int e = x - d + 1; e <= x; ++e)
-        if (in_hist(e, ii[r])) q |= 1 << ((d - 1) - (x - e));
-      out[(size_t)r] = q;
-    }
+    if (x >= 0)
+      for (int64_t r = 0; r < R; ++r) {
+        if (ii[r] < 0) continue;
+        int q = 0;
+        for (int b = 0; b < di[r]; ++b)
+          if (pending(x + 2 + b, r)) q |= 1 << b;
+        out[(size_t)r] = q;
+      }
   } else {
     auto& out = raw<int8_t>(v);
     out.assign((size_t)R, 0);
-    for (int64_t r = 0; r < R; ++r) {
-      if (ii[r] < 0 || x < 0) continue;
-      out[(size_t)r] = in_hist(x - di[r], ii[r]) ? 1 : 0;
-    }
+    if (x >= 0)
+      for (int64_t r = 0; r < R; ++r)
+        if (ii[r] >= 0 && pending(x + 1, r)) out[(size_t)r] = 1;
   }
 }
 
@@ -399,9 +406,6 @@ void Pop::init_new(ProcParam proc_param, PopParam pop_param, Database& db, vecto
   spk_list.alloc(N);
   spk_cnt.alloc(_dim_hcu);
   op_check(gsbn_i32_fill(spk_cnt.p, _dim_hcu, 0, active_stream()), "fill");
-  hist_list.alloc((size_t)GSBN_HIST * N);
-  hist_cnt.alloc((size_t)GSBN_HIST * _dim_hcu);
-  op_check(gsbn_i32_fill(hist_cnt.p, (int64_t)GSBN_HIST * _dim_hcu, 0, active_stream()), "fill");
   scratch.alloc(1);
   /* external spikes for debugging (Pop.cpp:67-110): lines step,pop,idx,... */
   Parser par(proc_param);
@@ -487,9 +491,16 @@ gsbn_pop_args Pop::args(int simstep, int exec, float prn, int lginp_idx, int wma
   }
   a.spk_list = spk_list.p;
   a.spk_cnt = spk_cnt.p;
-  if (write_hist) {
-    a.hist_out_list = hist_list.p;
-    a.hist_out_cnt = hist_cnt.p;
+  if (push_direct && !_out.empty()) {
+    vector<gsbn_fanout> outs;
+    for (Proj* q : _out) outs.push_back(q->fanout());
+    if (outs.size() != outs_host.size() ||
+        memcmp(outs.data(), outs_host.data(), outs.size() * sizeof(gsbn_fanout))) {
+      outs_host = outs;
+      outs_dev.upload(outs.data(), outs.size());
+    }
+    a.nproj_out = (int32_t)outs.size();
+    a.outs = outs_dev.p;
   }
   a.nproj_in = (int)_in.size();
   a.do_jxx = do_jxx ? 1 : 0;
@@ -676,13 +687,14 @@ void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
   rows_cnt.alloc(_dim_hcu);
   pend_cnt.alloc(_dim_hcu);
   act_stamp.alloc((size_t)R);
-  hist_list.alloc((size_t)GSBN_HIST * std::max(_nsrc, 1));
-  hist_cnt.alloc((size_t)GSBN_HIST * _ptr_src_pop->_dim_hcu);
+  W = (_dim_conn + 31) / 32;
+  const int64_t nfut = (int64_t)GSBN_FUT * _dim_hcu * W;
+  fut.alloc((size_t)nfut);
   op_check(gsbn_i32_fill(rows_cnt.p, _dim_hcu, 0, active_stream()), "fill");
   op_check(gsbn_i32_fill(pend_cnt.p, _dim_hcu, 0, active_stream()), "fill");
   op_check(gsbn_i32_fill(act_stamp.p, R, -1, active_stream()), "fill");
-  op_check(gsbn_i32_fill(hist_cnt.p, (int64_t)GSBN_HIST * _ptr_src_pop->_dim_hcu, 0,
-                         active_stream()), "fill");
+  op_check(gsbn_i32_fill(reinterpret_cast<int32_t*>(fut.p), nfut, 0, active_stream()), "fill");
+  _ptr_src_pop->_out.push_back(this);
 
   /* views over the device state */
   struct VB {
@@ -728,63 +740,43 @@ gsbn_pop_proj Proj::pop_desc() {
   d.kftj = _kftj;
   d.bgain = _bgain;
   d.eps = _eps;
-  d.hist_list = ring_list();
-  d.hist_cnt = ring_cnt();
-  d.Hsrc = _ptr_src_pop->_dim_hcu;
-  d.Msrc = _ptr_src_pop->_dim_mcu;
-  d.Nsrc = _nsrc;
-  d.dmin = dmin;
-  d.dmax = dmax;
-  d.inv_off = inv_off.p;
-  d.inv_slot = inv_slot.p;
-  d.di = _di->gpu_data();
+  d.fut = fut.p;
+  d.W = W;
   d.rows_list = rows_list.p;
   d.rows_cnt = rows_cnt.p;
   d.act_stamp = act_stamp.p;
   return d;
 }
 
-/* inverse connectivity per (destination HCU, source unit), slots ascending */
+/* rows of each source unit (ii == n), ascending, with their delays */
 void Proj::rebuild_fanout() {
-  const int H = _dim_hcu, C = _dim_conn;
+  const int64_t R = (int64_t)_dim_hcu * _dim_conn;
   const int* ii = _ii->cpu_data();
   const int* di = _di->cpu_data();
-  vector<int32_t> off((size_t)H * (_nsrc + 1), 0);
-  vector<int32_t> slots;
-  slots.reserve((size_t)H * C);
-  dmin = 32;
-  dmax = 0;
-  for (int h = 0; h < H; ++h) {
-    int32_t* o = off.data() + (size_t)h * (_nsrc + 1);
-    vector<int32_t> cnt(_nsrc, 0);
-    for (int k = 0; k < C; ++k) {
-      const int n = ii[(size_t)h * C + k];
-      if (n < 0) continue;
-      if (n >= _nsrc) throw Error(-2, "ii_" + std::to_string(_id) + ": source unit out of range");
-      const int d = di[(size_t)h * C + k];
-      if (d < 1 || d > 31) throw Error(-2, "di_" + std::to_string(_id) + ": delay must be in [1, 31]");
-      cnt[n]++;
-      dmin = std::min(dmin, d);
-      dmax = std::max(dmax, d);
-    }
-    const int base = (int)slots.size();
-    o[0] = base;
-    for (int n = 0; n < _nsrc; ++n) o[n + 1] = o[n] + cnt[n];
-    slots.resize(base + (size_t)(o[_nsrc] - base));
-    vector<int32_t> fill(o, o + _nsrc);
-    for (int k = 0; k < C; ++k) {
-      const int n = ii[(size_t)h * C + k];
-      if (n < 0) continue;
-      slots[fill[n]++] = k;
-    }
+  vector<int32_t> off((size_t)_nsrc + 1, 0);
+  for (int64_t r = 0; r < R; ++r) {
+    const int n = ii[r];
+    if (n < 0) continue;
+    if (n >= _nsrc) throw Error(-2, "ii_" + std::to_string(_id) + ": source unit out of range");
+    const int d = di[r];
+    if (d < 1 || d > 31) throw Error(-2, "di_" + std::to_string(_id) + ": delay must be in [1, 31]");
+    off[(size_t)n + 1]++;
   }
-  if (dmax == 0) {
-    dmin = 1;
-    dmax = 0; /* no connections: empty delay range */
+  for (int n = 0; n < _nsrc; ++n) off[(size_t)n + 1] += off[(size_t)n];
+  const size_t nnz = (size_t)off[(size_t)_nsrc];
+  vector<int32_t> rows(std::max<size_t>(nnz, 1), 0);
+  vector<uint8_t> del(std::max<size_t>(nnz, 1), 0);
+  vector<int32_t> fill(off.begin(), off.end() - 1);
+  for (int64_t r = 0; r < R; ++r) {
+    const int n = ii[r];
+    if (n < 0) continue;
+    const int32_t q = fill[(size_t)n]++;
+    rows[(size_t)q] = (int32_t)r;
+    del[(size_t)q] = (uint8_t)di[r];
   }
-  inv_off.upload(off.data(), off.size());
-  if (slots.empty()) slots.push_back(0);
-  inv_slot.upload(slots.data(), slots.size());
+  fwd_off.upload(off.data(), off.size());
+  fwd_row.upload(rows.data(), rows.size());
+  fwd_del.upload(del.data(), del.size());
 }
 
 void Proj::prepare(int simstep) {
@@ -900,12 +892,12 @@ void Proj::update_que(int exec) {
   last_exec = exec;
   if (direct_que) return; /* the population kernel already stored si of the next step */
   gsbn_que_args a;
+  memset(&a, 0, sizeof(a));
   a.Hsrc = _ptr_src_pop->_dim_hcu;
   a.Msrc = _ptr_src_pop->_dim_mcu;
   a.exec = exec;
   a.si = _si->gpu_data();
-  a.hist_list = hist_list.p;
-  a.hist_cnt = hist_cnt.p;
+  a.out = fanout();
   Timing::get().begin(4, active_stream());
   op_check(gsbn_proj_que(&a, active_stream()), "gsbn_proj_que");
   Timing::get().end(4, active_stream());
@@ -1084,13 +1076,13 @@ void ProcUpdLazy::update_gpu() {
   if (!_configured) {
     /* ProcExchangeSpike initialised after this procedure delivers
      * spike_<src> to si_<proj> once per executed step, so the queue push
-     * of step x sees the spikes of step x-1: the population kernel stores
-     * them in its history ring directly */
+     * of step x+1 is the spikes of step x: the population kernel pushes
+     * them itself */
     int direct = 0;
     _glv.geti("exchange-spike-direct", direct);
     for (auto* q : _list_proj) {
       q->direct_que = direct != 0;
-      if (q->direct_que) q->_ptr_src_pop->write_hist = true;
+      if (q->direct_que) q->_ptr_src_pop->push_direct = true;
     }
     _batch = _list_pop.size() > 1 || _list_proj.size() > 1;
     if (_batch) build_batch_maps();

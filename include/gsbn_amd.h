@@ -65,7 +65,7 @@ typedef struct {
   int32_t tij;
 } gsbn_syn16_rec;
 
-#define GSBN_HIST 64 /* exec steps kept in the spike queue ring (delays <= 31) */
+#define GSBN_FUT 64 /* slots of the pending-activation ring (delays <= 31) */
 #define GSBN_MAX_PROJ_IN 8
 #define GSBN_KP_SLOTS 4 /* distinct learning rates a batched projection can switch between */
 
@@ -91,23 +91,32 @@ typedef struct {
   float* ej;
   float* zj;
   float taupdt, ke, kzj, kepsc, kftj, bgain, eps;
-  /* spike queue ring, per source HCU: at the queue update of exec step e
-   * (slot = e % GSBN_HIST) source HCU hs had si set for the MCUs
-   * hist_list[(slot*Hsrc + hs)*Msrc + i], i < hist_cnt[slot*Hsrc + hs] */
-  const int32_t* hist_list;
-  const int32_t* hist_cnt;
-  int32_t Hsrc, Msrc, Nsrc, dmin, dmax;
-  /* inverse connectivity: for destination HCU h and source unit n the slots
-   * inv_slot[inv_off[h*(Nsrc+1)+n] .. inv_off[h*(Nsrc+1)+n+1]) hold ii == n */
-  const int32_t* inv_off;
-  const int32_t* inv_slot;
-  const int32_t* di;
+  /* pending activations of this projection's rows (see gsbn_fanout):
+   * the population kernel reads and clears slot exec % GSBN_FUT */
+  uint32_t* fut;
+  int32_t W; /* words per HCU and slot, (C + 31) / 32 */
   /* outputs: this step's active rows per HCU, ascending (the reference's
    * .ssi_<id>, Proj.cpp:621-630), and the exec stamp of each active row */
   int32_t* rows_list;
   int32_t* rows_cnt;
   int32_t* act_stamp;
 } gsbn_pop_proj;
+
+/* Delay queue in push form (Proj.cpp:645-664 with ProcExchangeSpike's
+ * one-step lag): a spike of source unit n at exec step s makes every row
+ * r = h*C + k with ii[r] == n active at exec step s + di[r] + 2.  Pending
+ * activations are a ring of bitmaps, one bit per row:
+ *   fut[((x % GSBN_FUT) * H + h) * W + (k >> 5)], bit k & 31,
+ * set for exec step x by the spike's push and cleared when step x reads it.
+ * fwd_off/fwd_row/fwd_del list the rows of each source unit (CSR over the
+ * source units, rows ascending) with their delays. */
+typedef struct {
+  const int32_t* fwd_off; /* [Nsrc + 1] */
+  const int32_t* fwd_row; /* r = h*C + k */
+  const uint8_t* fwd_del; /* di[r] */
+  uint32_t* fut;          /* ring of the destination projection */
+  int32_t H, C, W, pad;   /* destination HCUs, slots, words per HCU and slot */
+} gsbn_fanout;
 
 /* Fused population step: epsc finalisation, random draws, support, dsup,
  * WTA soft-max, spike sampling and counters (Pop.cpp:220-387, Pop.cu:17-108),
@@ -137,12 +146,12 @@ typedef struct {
   int32_t ext_nsteps;
   int32_t* spk_list; /* [H][M] spiking units of each HCU, ascending */
   int32_t* spk_cnt;  /* [H] */
-  /* optional spike-history ring of this population in the queue format: the
-   * spikes of exec step x are stored at slot (x+1) % GSBN_HIST, which is what
-   * the queue update of step x+1 sees when ProcExchangeSpike delivers
-   * spike_<pop> to si_<proj> once per step; NULL to skip */
-  int32_t* hist_out_list;
-  int32_t* hist_out_cnt;
+  /* outgoing projections whose queue push this kernel does itself: when
+   * ProcExchangeSpike delivers spike_<pop> to si_<proj> once per step, the
+   * push of step x+1 is this step's spikes, so they go straight into the
+   * rings (0 outgoing: the push is left to gsbn_proj_que) */
+  int32_t nproj_out, pad_out;
+  const gsbn_fanout* outs; /* device array */
   int32_t nproj_in, do_jxx;
   int32_t max_slots; /* largest C among the incoming projections (sizes the row bitmap) */
   const gsbn_pop_proj* projs; /* device array, index = position of the projection in the population */
@@ -233,13 +242,13 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
                            const int32_t* wg_hcu, int32_t nhcu, int32_t parts,
                            const gsbn_step_args* step, void* stream);
 
-/* Delay-queue push (Proj.cpp:645-664): record the source units with si > 0
- * for exec step `exec`, per source HCU.  Grid: one workgroup per source HCU. */
+/* Delay-queue push (Proj.cpp:645-664) from si: si holds the spikes of exec
+ * step exec - 1 (ProcExchangeSpike's copy), each source unit with si > 0
+ * pushes its rows.  Grid: one workgroup per source HCU. */
 typedef struct {
-  int32_t Hsrc, Msrc, exec;
+  int32_t Hsrc, Msrc, exec, pad;
   const int8_t* si;
-  int32_t* hist_list;
-  int32_t* hist_cnt;
+  gsbn_fanout out;
 } gsbn_que_args;
 int gsbn_proj_que(const gsbn_que_args* a, void* stream);
 
@@ -297,6 +306,50 @@ void gsbn_host_draw(uint64_t seed, uint32_t pop, uint32_t step, uint32_t unit, f
  * 1 logf, 2 Philox draw, 3 float division, 4 Wij, 5 float -> bfloat16 code,
  * 6 float -> binary16 code, 7 binary16 code -> float */
 int gsbn_device_math(int op, int64_t n, const void* in_host, void* out_host);
+
+/* ---- frozen-weight batched recall ----
+ * No single counterpart in the reference: B cues of one trained population
+ * run the reference's recall-mode dynamics (prn = 0: bj and Wij frozen,
+ * Proj.cpp:310-363 and 365-479; Pop.cpp:229-323) in lock-step, each from the
+ * population's initial state (Pop.cpp:59-60) with an empty delay queue and
+ * epsc = 0, cue b drawing with Philox key seed + b, population 0.
+ * GSBN_RECALL_EXACT adds float Wij of the active rows in the reference's
+ * ascending row order (a single-cue solver run with the same weights gives
+ * the same spikes); GSBN_RECALL_MFMA computes the support of all cues as
+ * the product of the 0/1 spike matrix and W in bfloat16 on the matrix cores
+ * with float accumulation (one delay for all rows required). */
+#define GSBN_RECALL_EXACT 0
+#define GSBN_RECALL_MFMA 1
+typedef struct {
+  int32_t H, M, C;     /* population of H HCUs x M MCUs; C slots per HCU of its recurrent projection */
+  const int32_t* ii;   /* [H*C] source unit of each slot, -1 when empty (host memory) */
+  const int32_t* di;   /* [H*C] delays (host memory) */
+  const float* wij;    /* [H*C*M] frozen weights (host memory) */
+  const float* bj;     /* [H*M] frozen bias of the projection (host memory) */
+  float taumdt, wtagain, maxfqdt, igain, wgain, lgbias, snoise; /* population parameters */
+  float kepsc;         /* support decay per step, dt / tauepsc */
+  float lg_on, lg_off; /* .lginp of a clamped MCU and of the others (log(1+eps), log(eps)) */
+  uint64_t seed;
+  int32_t batch;       /* cues per run */
+  int32_t mode;        /* GSBN_RECALL_EXACT or GSBN_RECALL_MFMA */
+} gsbn_recall_desc;
+typedef struct gsbn_recall gsbn_recall;
+gsbn_recall* gsbn_recall_new(const gsbn_recall_desc* d);
+void gsbn_recall_free(gsbn_recall* r);
+/* Run `steps` steps for the batch: clamp[b*H + h] = clamped MCU of HCU h in
+ * cue b or -1, wmask[b*H + h] the support mask (host memory).  Spikes of steps
+ * t > count_from are counted; out_idx/out_cnt[b*H + h] = first strict maximum
+ * of the counts and its count (-1 / 0 without spikes, ProcCheck/Pop.cpp:35-50).
+ * raster (optional, host, steps*batch*H*M int8) receives every step's spikes;
+ * support_ms (optional) the summed time of the support products (mfma). */
+int gsbn_recall_run(gsbn_recall* r, const int16_t* clamp, const float* wmask, int32_t steps,
+                    int32_t count_from, int32_t* out_idx, int32_t* out_cnt, int8_t* raster,
+                    double* support_ms);
+/* E[b][n] = beta E[b][n] + sum_k S[b][k] WT[n][k], S and WT bfloat16 with
+ * Kp columns (Kp % 64 == 0, zero padded), WT with ceil(N/128)*128 rows, E
+ * float [B][N]; device pointers. */
+int gsbn_spike_gemm(int32_t B, int32_t N, int32_t Kp, const uint16_t* S, const uint16_t* WT,
+                    float* E, float beta, void* stream);
 
 /* ---- solver level ---- */
 typedef struct gsbn_solver gsbn_solver;
