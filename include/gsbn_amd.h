@@ -330,7 +330,7 @@ typedef struct {
   float kepsc;         /* support decay per step, dt / tauepsc */
   float lg_on, lg_off; /* .lginp of a clamped MCU and of the others (log(1+eps), log(eps)) */
   uint64_t seed;
-  int32_t batch;       /* cues per run */
+  int32_t batch;       /* cues per run, <= 65535 */
   int32_t mode;        /* GSBN_RECALL_EXACT or GSBN_RECALL_MFMA */
 } gsbn_recall_desc;
 typedef struct gsbn_recall gsbn_recall;
@@ -346,8 +346,9 @@ int gsbn_recall_run(gsbn_recall* r, const int16_t* clamp, const float* wmask, in
                     int32_t count_from, int32_t* out_idx, int32_t* out_cnt, int8_t* raster,
                     double* support_ms);
 /* E[b][n] = beta E[b][n] + sum_k S[b][k] WT[n][k], S and WT bfloat16 with
- * Kp columns (Kp % 64 == 0, zero padded), WT with ceil(N/128)*128 rows, E
- * float [B][N]; device pointers. */
+ * Kp columns (Kp % 64 == 0, zero padded), S with ceil(B/128)*128 rows and WT
+ * with ceil(N/128)*128 rows (padding rows zero), E float [B][N]; device
+ * pointers. */
 int gsbn_spike_gemm(int32_t B, int32_t N, int32_t Kp, const uint16_t* S, const uint16_t* WT,
                     float* E, float beta, void* stream);
 

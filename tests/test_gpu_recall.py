@@ -1,0 +1,154 @@
+"""Frozen-weight batched recall on the GPU (-m gpu).
+
+* the support product on the matrix cores against numpy: exactly for
+  integer weights (every partial sum is exact in float), within the float
+  summation bound for random bfloat16 weights, including ragged tile edges;
+* the exact mode against the oracle: for several cues, the oracle runs one
+  cue in recall mode with the same frozen weights and seed, and every step's
+  spikes must be identical;
+* the bfloat16 matrix-core mode against the exact mode: recall outcomes agree.
+"""
+
+import ctypes
+import tempfile
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from gsbn_amd import Solver, configs, lib
+from gsbn_amd import recall as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _gemm(S, WT, E, beta):
+    L = lib()
+    R._setup(L)
+    B, Kp = S.shape
+    N = E.shape[1]
+    dev = torch.device("cuda:0")
+    Bp = (B + 127) // 128 * 128
+    Np = (N + 127) // 128 * 128
+    s = torch.zeros((Bp, Kp), dtype=torch.int16, device=dev)
+    w = torch.zeros((Np, Kp), dtype=torch.int16, device=dev)
+    s[:B] = torch.from_numpy(S.view(np.int16)).to(dev)
+    w[:N] = torch.from_numpy(WT.view(np.int16)).to(dev)
+    e = torch.from_numpy(E.copy()).to(dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    rc = L.gsbn_spike_gemm(B, N, Kp, s.data_ptr(), w.data_ptr(), e.data_ptr(), beta, stream)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return e.cpu().numpy()
+
+
+def _bf16_bits(x):
+    return torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(torch.bfloat16).view(
+        torch.int16).numpy().view(np.uint16)
+
+
+@pytest.mark.parametrize("B,N,K", [(128, 128, 64), (200, 300, 500), (256, 1000, 960)])
+def test_spike_gemm_integer_weights_exact(B, N, K):
+    rng = np.random.default_rng(B + N + K)
+    Kp = (K + 63) // 64 * 64
+    S = np.zeros((B, Kp), np.uint16)
+    S[:, :K] = np.where(rng.random((B, K)) < 0.05, 0x3F80, 0)
+    W = rng.integers(-8, 9, size=(N, K)).astype(np.float32)
+    WT = np.zeros((N, Kp), np.uint16)
+    WT[:, :K] = _bf16_bits(W)
+    E = (rng.integers(-1024, 1024, size=(B, N)) / 1024.0).astype(np.float32)
+    got = _gemm(S, WT, E, 0.5)
+    want = (0.5 * E.astype(np.float64) + (S[:, :K] != 0).astype(np.float64) @ W.T.astype(np.float64))
+    assert np.array_equal(got, want.astype(np.float32))
+
+
+def test_spike_gemm_random_bf16_weights():
+    rng = np.random.default_rng(7)
+    B, N, K = 300, 700, 1300
+    Kp = (K + 63) // 64 * 64
+    S = np.zeros((B, Kp), np.uint16)
+    spk = rng.random((B, K)) < 0.02
+    S[:, :K] = np.where(spk, 0x3F80, 0)
+    Wf = rng.normal(0, 1, size=(N, K)).astype(np.float32)
+    WT = np.zeros((N, Kp), np.uint16)
+    WT[:, :K] = _bf16_bits(Wf)
+    Wb = (WT[:, :K].astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    E = rng.normal(0, 1, size=(B, N)).astype(np.float32)
+    beta = np.float32(0.95)
+    got = _gemm(S, WT, E, float(beta)).astype(np.float64)
+    prod = spk.astype(np.float64) @ Wb.T
+    want = np.float64(beta) * E.astype(np.float64) + prod
+    bound = 2 ** -22 * (np.abs(E) + spk.astype(np.float64) @ np.abs(Wb).T + 1e-30) * (K + 2)
+    assert np.all(np.abs(got - want) <= bound)
+
+
+def _train(tmp_path, spec):
+    """Run the solver through training into the first recall step (after the
+    sweep of the learning-rate switch) and return it."""
+    s = Solver(configs.write(spec, str(tmp_path)))
+    configs.apply_conn_arrays(s, spec)
+    for _ in range(100000):
+        s.step(1)
+        if s.glob("cycle-flag") == 1 and s.glob("prn", "f") == 0.0:
+            break
+    return s
+
+
+@pytest.fixture(scope="module")
+def trained10(tmp_path_factory):
+    spec = configs.config0()
+    s = _train(tmp_path_factory.mktemp("train10"), spec)
+    yield spec, s
+    s.close()
+
+
+def test_recall_exact_matches_oracle(trained10):
+    spec, s = trained10
+    pats = spec.stim[:10].astype(np.int64)
+    B, T = 6, 60
+    clamp, wmask, want = R.cues(pats, B, 0.4, seed=3)
+    rc = R.Recall.from_solver(s, spec, B, R.EXACT, seed=1234)
+    idx, cnt, ras, _ = rc.run(clamp, wmask, T, count_from=0, raster=True)
+    ii, di = s.vector("ii_0"), s.vector("di_0")
+    wij, bj = s.vector("wij_0"), s.vector("bj_0")
+    for b in range(B):
+        ospec = R.oracle_recall_spec(spec, clamp[b], wmask[b], T, 1234 + b, ii, di)
+        net = O.OracleNet(ospec)
+        net.write("wij_0", wij)
+        net.write("bj_0", bj)
+        counts = np.zeros(spec.pops[0].hcu * spec.pops[0].mcu, np.int64)
+        for t in range(T):
+            assert net.step() == 1
+            sp = net.vector("spike_0")
+            assert np.array_equal(sp, ras[t, b]), "cue %d diverges at step %d" % (b, t + 1)
+            counts += sp
+        H, M = spec.pops[0].hcu, spec.pops[0].mcu
+        c = counts.reshape(H, M)
+        for h in range(H):
+            mi, mc = -1, 0
+            for j in range(M):
+                if c[h, j] > mc:
+                    mi, mc = j, int(c[h, j])
+            assert (idx[b, h], cnt[b, h]) == (mi, mc)
+    rc.close()
+
+
+def test_recall_mfma_agrees_with_exact(trained10):
+    spec, s = trained10
+    pats = spec.stim[:10].astype(np.int64)
+    B, T = 512, 60
+    clamp, wmask, want = R.cues(pats, B, 0.4, seed=5)
+    out = {}
+    for mode in (R.EXACT, R.MFMA):
+        rc = R.Recall.from_solver(s, spec, B, mode, seed=99)
+        idx, cnt, _, ms = rc.run(clamp, wmask, T, count_from=T // 2)
+        out[mode] = idx
+        rc.close()
+    ok_exact = R.recall_correct(out[R.EXACT], want)
+    ok_mfma = R.recall_correct(out[R.MFMA], want)
+    agree = np.mean(ok_exact == ok_mfma)
+    winners = np.mean(out[R.EXACT] == out[R.MFMA])
+    print("recall exact %.3f mfma %.3f outcome agreement %.3f winner agreement %.3f"
+          % (ok_exact.mean(), ok_mfma.mean(), agree, winners))
+    assert agree >= 0.9 and winners >= 0.9
