@@ -48,6 +48,7 @@ METRIC = ("synapse-trace updates/sec + sim steps/sec at 100×100 HCU×MCU, plain
 # (column); 16-bit forms move 2-byte pij/eij
 BYTES_PER_SYN_UPDATE = {"fp32": 44, "bf16": 36, "fp16": 36}
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense bf16
 
 
 def workload(name: str):
@@ -126,6 +127,7 @@ def measure(spec, steps, warmup, step_times=0):
         t1 = time.perf_counter()
         launches, ms = s.timing(0)
         w1 = s.work()
+        parts = s.glob("rowcol-parts")
         step_ms = None
         if step_times:
             # per-step wall times (each step synchronised) for the p50/p99 report
@@ -151,7 +153,96 @@ def measure(spec, steps, warmup, step_times=0):
             "updates_per_s": (rows + cols + sweep) / el, "steps_per_s": done / el,
             "ms_per_step": 1e3 * el / max(done, 1), "init_s": t_init, "step_ms": step_ms,
             "rc_launches": rc_launch, "rc_avg_ms": avg_ms, "rc_bytes_per_launch": bytes_per_launch,
-            "rc_achieved_gbs": achieved}
+            "rc_achieved_gbs": achieved, "rc_parts": parts}
+
+
+def copy_calibration(mib: int):
+    """Five copies of `mib` MiB by gsbn_device_copy (buffers from torch)."""
+    import ctypes
+    import torch
+    from gsbn_amd import lib
+    L = lib()
+    L.gsbn_device_copy.restype = ctypes.c_int
+    L.gsbn_device_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.c_void_p]
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(5):
+        if L.gsbn_device_copy(dst.data_ptr(), src.data_ptr(), n, stream) != 0:
+            raise SystemExit("gsbn_device_copy failed")
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def batched(steps: int, warmup: int) -> dict:
+    """The 512-network form: aggregate and per-network step rates."""
+    spec, desc = workload("batch512")
+    m = measure(spec, steps, warmup, step_times=100)
+    n = len(spec.pops)
+    return {"workload": desc, "networks": n, "steps": m["done"],
+            "steps_per_sec": m["steps_per_s"], "network_steps_per_sec": n * m["steps_per_s"],
+            "synapse_updates_per_sec": m["updates_per_s"],
+            "step_ms_p50": m["step_ms"]["p50"], "step_ms_p99": m["step_ms"]["p99"],
+            "rowcol_avg_launch_ms": m["rc_avg_ms"], "rowcol_achieved_gbs": m["rc_achieved_gbs"],
+            "rowcol_workgroups_per_hcu": m["rc_parts"]}
+
+
+def frozen_recall(batch: int = 8192, steps: int = 50, clamp_frac: float = 0.4) -> dict:
+    """Train the 100x100 network (its training phase through the sweep of the
+    first recall step), freeze W and bj, run `batch` cues for `steps` steps
+    with the exact float support and with the bf16 matrix-core support."""
+    from gsbn_amd import Solver, configs
+    from gsbn_amd import recall as R
+    spec = configs.config1()
+    with tempfile.TemporaryDirectory() as d:
+        s = Solver(configs.write(spec, d))
+        configs.apply_conn_arrays(s, spec)
+        trained = 0
+        while True:
+            s.step(1)
+            trained += 1
+            if s.glob("cycle-flag") == 1 and s.glob("prn", "f") == 0.0:
+                break
+        pats = np.asarray(spec.stim[:100], np.int64)
+        clamp, wmask, want = R.cues(pats, batch, clamp_frac, seed=11)
+        out = {}
+        for name, mode in (("exact", R.EXACT), ("mfma", R.MFMA)):
+            rc = R.Recall.from_solver(s, spec, batch, mode, seed=77)
+            rc.run(clamp, wmask, 4)  # first launch of every kernel, untimed
+            t0 = time.perf_counter()
+            idx, cnt, _, _ = rc.run(clamp, wmask, steps, count_from=steps // 2)
+            el = time.perf_counter() - t0
+            ok = R.recall_correct(idx, want)
+            out[name] = {"cues_per_sec": batch / el, "seconds": el, "recall_correct": float(ok.mean()),
+                         "hcu_winner_accuracy": float(np.mean(idx == want)), "idx": idx}
+            if mode == R.MFMA:
+                # support products run from step delay+3 on
+                _, _, _, sup_ms = rc.run(clamp, wmask, steps, count_from=steps // 2,
+                                         time_support=True)
+                N = spec.total_units
+                nprod = steps - 3
+                flops = 2.0 * batch * N * N * nprod
+                out[name]["support_ms"] = sup_ms
+                out[name]["mfma_tflops"] = flops / (sup_ms * 1e-3) / 1e12 if sup_ms > 0 else None
+            rc.close()
+        s.close()
+    agree = float(np.mean(R.recall_correct(out["exact"]["idx"], want) ==
+                          R.recall_correct(out["mfma"]["idx"], want)))
+    winners = float(np.mean(out["exact"]["idx"] == out["mfma"]["idx"]))
+    for v in out.values():
+        v.pop("idx")
+    mf = out["mfma"]
+    return {"workload": "100x100 network trained %d steps, W and bj frozen (bf16 for mfma), "
+                        "%d cues (%.0f%% of HCUs clamped) x %d lock-step steps"
+                        % (trained, batch, 100 * clamp_frac, steps),
+            "exact": out["exact"], "mfma": mf, "outcome_agreement": agree,
+            "winner_agreement": winners,
+            "mfma_roofline": {"bound": "mfma", "achieved": mf.get("mfma_tflops"),
+                              "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": (mf["mfma_tflops"] / MFMA_BF16_PEAK_TFLOPS
+                                       if mf.get("mfma_tflops") else None)}}
 
 
 def main():
@@ -164,6 +255,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the bf16 / fp16 trace-storage runs of the same workload")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 512-network form")
+    ap.add_argument("--no-recall", action="store_true", help="skip the frozen-weight recall form")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB five times with the library's copy kernel "
+                         "(reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--step-times", type=int, default=0,
                     help="after the timed region, time this many synchronised single steps")
@@ -175,6 +271,8 @@ def main():
               "(see DESIGN.md). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     spec, desc = workload(args.workload)
     m = measure(spec, args.steps, args.warmup, args.step_times)
     traffic = None
@@ -208,7 +306,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "rowcol_kernel", "avg_launch_ms": m["rc_avg_ms"],
-                     "launches": m["rc_launches"], "bytes_per_launch": m["rc_bytes_per_launch"]},
+                     "launches": m["rc_launches"], "bytes_per_launch": m["rc_bytes_per_launch"],
+                     "workgroups_per_hcu": m["rc_parts"]},
     }
     if m["step_ms"] is not None:
         line["step_ms"] = m["step_ms"]
@@ -227,6 +326,12 @@ def main():
                            "rowcol_achieved_gbs": f["rc_achieved_gbs"],
                            "bytes_per_update": BYTES_PER_SYN_UPDATE[form]}
         line["forms"] = forms
+    if args.workload == "100x100" and not args.no_batched:
+        line["batched"] = batched(300, 100)
+        line["batched"]["speedup_network_steps_per_sec"] = (
+            line["batched"]["network_steps_per_sec"] / m["steps_per_s"])
+    if args.workload == "100x100" and not args.no_recall:
+        line["frozen_recall"] = frozen_recall()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line))

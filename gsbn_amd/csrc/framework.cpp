@@ -255,27 +255,32 @@ hipEvent_t Timing::take() {
   return e;
 }
 
+/* begin/end bracket one kernel launch: the launch records the events as
+ * part of its dispatch; if nothing was launched they are recorded here */
 void Timing::begin(int kind, hipStream_t s) {
+  (void)s;
   if (!_on || !((_mask >> kind) & 1)) return;
   Rec r;
   r.kind = kind;
   r.a = take();
-  r.b = nullptr;
-  hip_check(hipEventRecord(r.a, s), "hipEventRecord");
+  r.b = take();
+  set_launch_events(r.a, r.b);
   _recs.push_back(r);
 }
 
 void Timing::end(int kind, hipStream_t s) {
   if (!_on || !((_mask >> kind) & 1) || _recs.empty()) return;
   Rec& r = _recs.back();
-  if (r.kind != kind || r.b) return;
-  r.b = take();
-  hip_check(hipEventRecord(r.b, s), "hipEventRecord");
+  if (r.kind != kind) return;
+  if (launch_events_pending()) {
+    set_launch_events(nullptr, nullptr);
+    hip_check(hipEventRecord(r.a, s), "hipEventRecord");
+    hip_check(hipEventRecord(r.b, s), "hipEventRecord");
+  }
 }
 
 void Timing::collect(int64_t* launches, double* ms, int kinds) {
   for (auto& r : _recs) {
-    if (!r.b) continue;
     hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
     float t = 0.0f;
     hip_check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
@@ -294,10 +299,11 @@ void Timing::collect(int64_t* launches, double* ms, int kinds) {
 }
 
 void Timing::reset() {
+  set_launch_events(nullptr, nullptr);
   for (auto& r : _recs) {
-    if (r.b) (void)hipEventSynchronize(r.b);
+    (void)hipEventSynchronize(r.b);
     _pool.push_back(r.a);
-    if (r.b) _pool.push_back(r.b);
+    _pool.push_back(r.b);
   }
   _recs.clear();
   for (int k = 0; k < 8; ++k) {

@@ -444,6 +444,10 @@ class Solver {
 };
 
 /* kernel timing with HIP events on the solver stream */
+/* kernels.hip: the next timed kernel launch records these events itself */
+void set_launch_events(hipEvent_t a, hipEvent_t b);
+bool launch_events_pending();
+
 class Timing {
  public:
   static Timing& get();

@@ -18,6 +18,7 @@
  * exp/log come from detmath.h.  The file must be compiled with
  * -ffp-contract=off.
  */
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -39,9 +40,33 @@ static_assert(sizeof(gsbn_syn16_rec) == 16 && sizeof(gsbn_syn16_t) == 16,
 #define ALL_THREADS 256
 #define ALL_ROWS 16
 
+namespace gsbn {
+/* events for the next timed launch: recorded by the kernel's own dispatch
+ * (hipExtLaunchKernelGGL), so that their difference is the kernel's time
+ * without the gaps that separate event packets add */
+thread_local hipEvent_t tl_ev_start = nullptr, tl_ev_stop = nullptr;
+void set_launch_events(hipEvent_t a, hipEvent_t b) {
+  tl_ev_start = a;
+  tl_ev_stop = b;
+}
+bool launch_events_pending() { return tl_ev_start != nullptr; }
+}  // namespace gsbn
+
 namespace {
 
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
+
+/* launch that takes the pending timing events, if any */
+template <typename K, typename... Args>
+inline void launch_timed(K kernel, dim3 g, dim3 b, size_t sh, hipStream_t s, Args... args) {
+  if (gsbn::tl_ev_start) {
+    hipEvent_t e0 = gsbn::tl_ev_start, e1 = gsbn::tl_ev_stop;
+    gsbn::tl_ev_start = gsbn::tl_ev_stop = nullptr;
+    hipExtLaunchKernelGGL(kernel, g, b, sh, s, e0, e1, 0, args...);
+  } else {
+    hipLaunchKernelGGL(kernel, g, b, sh, s, args...);
+  }
+}
 
 inline int launch_status() {
   hipError_t e = hipGetLastError();
@@ -144,21 +169,72 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   const bool live = j < M;
   const int idx = h * M + j;
   const int64_t N = (int64_t)a.H * M;
+  const int nin = a.nproj_in;
 
-  if (j < a.nproj_in) sh_pend[j] = a.projs[j].pend_cnt[h];
+  /* 0. every load that does not depend on this step's results is issued
+   * here, so that their latencies overlap (the first incoming projection's
+   * state and ring word; further projections load where they are used) */
+  float dsup_old = 0.0f, lg = 0.0f, wmv = 0.0f, e0 = 0.0f, bj0 = 0.0f, pj0 = 0.0f, ej0 = 0.0f,
+        zj0 = 0.0f;
+  int32_t counter_old = 0;
+  uint32_t fut0 = 0u;
+  int pend0 = 0;
+  if (live) {
+    dsup_old = a.dsup[idx];
+    lg = a.lginp[idx];
+    wmv = a.wmask[h];
+    counter_old = a.counter[idx];
+  }
+  if (nin > 0) {
+    const gsbn_pop_proj& p0 = a.projs[0];
+    pend0 = p0.pend_cnt[h];
+    if (live) {
+      e0 = a.epsc[idx];
+      bj0 = a.bj[idx];
+      pj0 = p0.pj[idx];
+      ej0 = p0.ej[idx];
+      zj0 = p0.zj[idx];
+    }
+    if (j < p0.W) fut0 = p0.fut[((int64_t)(a.exec % GSBN_FUT) * a.H + h) * p0.W + j];
+  }
+  if (j > 0 && j < nin) sh_pend[j] = a.projs[j].pend_cnt[h];
   __syncthreads();
 
-  /* 1. pending support contributions of the previous step's row updates */
+  /* 1. pending support contributions of the previous step's row updates,
+   * added in ascending row order; four loads in flight at a time */
   float efin[GSBN_MAX_PROJ_IN];
-  for (int m = 0; m < a.nproj_in; ++m) {
-    efin[m] = live ? finalized_epsc(a, a.projs[m], m, h, j, sh_pend[m]) : 0.0f;
-    if (live) a.epsc[m * N + idx] = efin[m];
+  for (int m = 0; m < nin; ++m) {
+    if (!live) {
+      efin[m] = 0.0f;
+      continue;
+    }
+    const gsbn_pop_proj& p = a.projs[m];
+    const int cnt = m == 0 ? pend0 : sh_pend[m];
+    float e = m == 0 ? e0 : a.epsc[m * N + idx];
+    const float* c = p.contrib + ((int64_t)h * p.C) * M + j;
+    for (int k = 0; k < cnt; k += 4) {
+      float v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = k + q < cnt ? c[(int64_t)(k + q) * M] : 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (k + q < cnt && gsbn_fbits(v[q]) != GSBN_NO_CONTRIB_BITS) e = e + v[q];
+    }
+    efin[m] = e;
+    a.epsc[m * N + idx] = e;
   }
 
-  /* 2. random draws, support, dsup */
+  /* 2. random draws, support, dsup (update_sup_kernel_1_cpu, Pop.cpp:229-261) */
   float u = 0.0f, xi = 0.0f, dsup = 0.0f;
   if (live) {
-    dsup = unit_dsup(a, h, j, efin, &u, &xi);
+    float z;
+    gsbn_draw_unit(a.seed, (uint32_t)a.pop_id, (uint32_t)a.simstep, (uint32_t)idx, &u, &z);
+    xi = a.snoise * z;
+    float wsup = 0.0f;
+    for (int m = 0; m < nin; ++m) wsup += (m == 0 ? bj0 : a.bj[m * N + idx]) + efin[m];
+    float sup = a.lgbias + a.igain * lg + xi;
+    sup += (a.wgain * wmv) * wsup;
+    dsup = dsup_old + (sup - dsup_old) * a.taumdt;
     a.dsup[idx] = dsup;
     a.rnd_u[idx] = u;
     a.rnd_n[idx] = xi;
@@ -217,7 +293,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   if (live) {
     a.act[idx] = act;
     spk = (u < act * a.maxfqdt) ? 1 : 0;
-    a.counter[idx] += spk;
+    a.counter[idx] = counter_old + spk;
   }
   if (a.ext_off) {
     sh_ext[j] = 0;
@@ -256,10 +332,11 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   /* 6. postsynaptic traces and bias for every incoming projection
    * (update_jxx_kernel_cpu, Proj.cpp:310-363) */
   if (a.do_jxx && live) {
-    for (int m = 0; m < a.nproj_in; ++m) {
+    for (int m = 0; m < nin; ++m) {
       const gsbn_pop_proj& p = a.projs[m];
       const float kp = p.taupdt * a.prn;
-      float pj = p.pj[idx], ej = p.ej[idx], zj = p.zj[idx];
+      float pj = m == 0 ? pj0 : p.pj[idx], ej = m == 0 ? ej0 : p.ej[idx],
+            zj = m == 0 ? zj0 : p.zj[idx];
       a.epsc[m * N + idx] = efin[m] * (1 - p.kepsc);
       pj += (ej - pj) * kp;
       ej += (zj - ej) * p.ke;
@@ -280,7 +357,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
     const int W = p.W;
     uint32_t* words = p.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * W;
     for (int w = j; w < W; w += blockDim.x) {
-      const uint32_t v = words[w];
+      const uint32_t v = (m == 0 && w == j) ? fut0 : words[w];
       sh_bitmap[w] = v;
       if (v) words[w] = 0u;
     }
@@ -783,6 +860,14 @@ __global__ void row_fill_kernel(gsbn_row_rec* row, int64_t n, gsbn_row_rec v) {
     row[i] = v;
 }
 
+/* plain streaming copy, 16 bytes per lane and iteration (the reference
+ * point for the memory-traffic counters) */
+__global__ void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, int64_t n16) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void i32_fill_kernel(int32_t* p, int64_t n, int32_t v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -898,7 +983,7 @@ int gsbn_pop_update(const gsbn_pop_args* a, void* stream) {
     int rc = launch_status();
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(pop_kernel, dim3(a->H), dim3(threads), lds, s, *a);
+  launch_timed(pop_kernel, dim3(a->H), dim3(threads), lds, s, *a);
   return launch_status();
 }
 
@@ -921,13 +1006,13 @@ int gsbn_proj_all(const gsbn_all_args* a, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   switch (a->prec) {
     case GSBN_PREC_F32:
-      hipLaunchKernelGGL(all_kernel<GSBN_PREC_F32>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      launch_timed(all_kernel<GSBN_PREC_F32>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
       break;
     case GSBN_PREC_BF16:
-      hipLaunchKernelGGL(all_kernel<GSBN_PREC_BF16>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      launch_timed(all_kernel<GSBN_PREC_BF16>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
       break;
     default:
-      hipLaunchKernelGGL(all_kernel<GSBN_PREC_F16>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      launch_timed(all_kernel<GSBN_PREC_F16>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
       break;
   }
   return launch_status();
@@ -936,7 +1021,7 @@ int gsbn_proj_all(const gsbn_all_args* a, void* stream) {
 int gsbn_proj_jxx(const gsbn_jxx_args* a, void* stream) {
   if (!a || a->N < 0) return GSBN_ERR_ARG;
   if (a->N == 0) return 0;
-  hipLaunchKernelGGL(jxx_kernel, dim3(grid_for(a->N, 256, 4096)), dim3(256), 0,
+  launch_timed(jxx_kernel, dim3(grid_for(a->N, 256, 4096)), dim3(256), 0,
                      (hipStream_t)stream, *a);
   return launch_status();
 }
@@ -955,13 +1040,13 @@ int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   switch (b.prec) {
     case GSBN_PREC_F32:
-      hipLaunchKernelGGL(rowcol_kernel<GSBN_PREC_F32>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
+      launch_timed(rowcol_kernel<GSBN_PREC_F32>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
       break;
     case GSBN_PREC_BF16:
-      hipLaunchKernelGGL(rowcol_kernel<GSBN_PREC_BF16>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
+      launch_timed(rowcol_kernel<GSBN_PREC_BF16>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
       break;
     default:
-      hipLaunchKernelGGL(rowcol_kernel<GSBN_PREC_F16>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
+      launch_timed(rowcol_kernel<GSBN_PREC_F16>, dim3((unsigned)grid), dim3(RC_THREADS), 0, s, b);
       break;
   }
   return launch_status();
@@ -982,7 +1067,7 @@ int gsbn_pop_update_batch(const gsbn_pop_args* pops_dev, int32_t npop, const int
     int rc = launch_status();
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(pop_kernel_batch, dim3(nwg), dim3(threads), lds, s, pops_dev, wg_pop, wg_hcu,
+  launch_timed(pop_kernel_batch, dim3(nwg), dim3(threads), lds, s, pops_dev, wg_pop, wg_hcu,
                      *step);
   return launch_status();
 }
@@ -1000,15 +1085,15 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
   const dim3 g((unsigned)grid), b(RC_THREADS);
   switch (step->prec) {
     case GSBN_PREC_F32:
-      hipLaunchKernelGGL(rowcol_kernel_batch<GSBN_PREC_F32>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+      launch_timed(rowcol_kernel_batch<GSBN_PREC_F32>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
                          parts, *step);
       break;
     case GSBN_PREC_BF16:
-      hipLaunchKernelGGL(rowcol_kernel_batch<GSBN_PREC_BF16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+      launch_timed(rowcol_kernel_batch<GSBN_PREC_BF16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
                          parts, *step);
       break;
     default:
-      hipLaunchKernelGGL(rowcol_kernel_batch<GSBN_PREC_F16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+      launch_timed(rowcol_kernel_batch<GSBN_PREC_F16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
                          parts, *step);
       break;
   }
@@ -1021,7 +1106,7 @@ int gsbn_proj_que(const gsbn_que_args* a, void* stream) {
       a->out.W != (a->out.C + 31) / 32)
     return GSBN_ERR_ARG;
   const int threads = a->Msrc >= 1024 ? 1024 : ((a->Msrc + 63) / 64) * 64;
-  hipLaunchKernelGGL(que_kernel, dim3(a->Hsrc), dim3(threads), 0, (hipStream_t)stream, *a);
+  launch_timed(que_kernel, dim3(a->Hsrc), dim3(threads), 0, (hipStream_t)stream, *a);
   return launch_status();
 }
 
@@ -1109,6 +1194,16 @@ int gsbn_syn16_fill(gsbn_syn16_rec* syn, int64_t n, gsbn_syn16_rec v, void* stre
   if (n == 0) return 0;
   hipLaunchKernelGGL(syn16_fill_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
                      (hipStream_t)stream, syn, n, v);
+  return launch_status();
+}
+
+int gsbn_device_copy(void* dst, const void* src, int64_t bytes, void* stream) {
+  if (bytes < 0 || bytes % 16 || (bytes && (!dst || !src)) ||
+      ((uintptr_t)dst | (uintptr_t)src) % 16)
+    return GSBN_ERR_ARG;
+  if (bytes == 0) return 0;
+  hipLaunchKernelGGL(copy_kernel, dim3(grid_for(bytes / 16, 256, 8192)), dim3(256), 0,
+                     (hipStream_t)stream, (uint4*)dst, (const uint4*)src, bytes / 16);
   return launch_status();
 }
 

@@ -946,12 +946,66 @@ class ProcUpdLazy : public ProcedureBase {
   DevBuf<gsbn_rowcol_args> _proj_dev;
   DevBuf<int32_t> _wg_pop, _wg_pop_hcu, _wg_proj, _wg_proj_hcu;
   int _pop_threads = 64, _max_slots = 0, _any_quirk = 0, _nwg_pop = 0, _nhcu_proj = 0, _parts = 1;
+  /* row/column work split: workgroups per destination HCU.  Fixed by argi
+   * rowcol-parts, or chosen online: from step kTuneStart on, each candidate
+   * is used for kTuneSamples executed steps in turn (results do not depend
+   * on the split) and the one with the smallest median kernel time is kept */
+  static constexpr int kTuneStart = 20, kTuneSamples = 6;
+  int _parts_fixed = 0;
+  bool _tuned = false;
+  int _tune_pos = 0;
+  vector<int> _cand;
+  vector<hipEvent_t> _tev;
+  int rowcol_parts(hipStream_t s, bool* timed);
+  void finish_tuning();
 };
 REGISTERIMPL(ProcUpdLazy);
 
 ProcUpdLazy::~ProcUpdLazy() {
+  for (hipEvent_t e : _tev) (void)hipEventDestroy(e);
   for (auto* p : _list_proj) delete p;
   for (auto* p : _list_pop) delete p;
+}
+
+/* parts for this step's row/column launch; *timed when the launch is a
+ * tuning sample (events _tev[2i], _tev[2i+1] are to be recorded around it) */
+int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
+  *timed = false;
+  if (_parts_fixed > 0) return _parts_fixed;
+  if (_tuned || execs < kTuneStart) return _parts;
+  if (_cand.empty()) {
+    for (int c : {1, 2, 4, 8, 16, 32})
+      if ((int64_t)c * std::max(_nhcu_proj, 1) <= 262144) _cand.push_back(c);
+    if (_cand.empty()) _cand.push_back(1);
+    _tev.resize(2 * _cand.size() * kTuneSamples);
+    for (auto& e : _tev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  }
+  (void)s;
+  *timed = true;
+  return _cand[_tune_pos % _cand.size()];
+}
+
+void ProcUpdLazy::finish_tuning() {
+  const size_t nc = _cand.size();
+  hip_check(hipEventSynchronize(_tev.back()), "hipEventSynchronize");
+  float best = 0.0f;
+  for (size_t c = 0; c < nc; ++c) {
+    vector<float> t;
+    for (int k = 0; k < kTuneSamples; ++k) {
+      const size_t i = (size_t)k * nc + c;
+      float ms = 0.0f;
+      hip_check(hipEventElapsedTime(&ms, _tev[2 * i], _tev[2 * i + 1]), "hipEventElapsedTime");
+      t.push_back(ms);
+    }
+    std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+    const float med = t[t.size() / 2];
+    if (c == 0 || med < best) {
+      best = med;
+      _parts = _cand[c];
+    }
+  }
+  _tuned = true;
+  _glv.puti("rowcol-parts", _parts);
 }
 
 void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
@@ -963,6 +1017,9 @@ void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
   par.argi("seed-hi", hi);
   _seed = (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 31);
   par.argi("wta-pop-max", _wta_pop_max);
+  par.argi("rowcol-parts", _parts_fixed);
+  if (_parts_fixed < 0 || _parts_fixed > 1024)
+    throw Error(-2, "ProcUpdLazy: rowcol-parts must be in [0, 1024]");
   int hcu_cnt = 0, mcu_cnt = 0;
   for (auto& pp : net_param.pop_param) {
     for (uint32_t j = 0; j < pp.pop_num; j++) {
@@ -1040,7 +1097,6 @@ void ProcUpdLazy::build_batch_maps() {
   }
   _nwg_pop = (int)wp.size();
   _nhcu_proj = (int)qp.size();
-  _parts = std::max(1, std::min(32, 1536 / std::max(_nhcu_proj, 1)));
   _wg_pop.upload(wp.data(), wp.size());
   _wg_pop_hcu.upload(wh.data(), wh.size());
   if (!qp.empty()) {
@@ -1086,6 +1142,11 @@ void ProcUpdLazy::update_gpu() {
     }
     _batch = _list_pop.size() > 1 || _list_proj.size() > 1;
     if (_batch) build_batch_maps();
+    _nhcu_proj = 0;
+    for (auto* q : _list_proj) _nhcu_proj += q->_dim_hcu;
+    /* starting split until the tuning has run: about 1536 workgroups */
+    _parts = std::max(1, std::min(32, 1536 / std::max(_nhcu_proj, 1)));
+    _glv.puti("rowcol-parts", _parts_fixed > 0 ? _parts_fixed : _parts);
     _configured = true;
     _args_dirty = true;
   }
@@ -1143,9 +1204,13 @@ void ProcUpdLazy::update_gpu() {
     for (auto* q : _list_proj) q->update_jxx(prn);
   }
   if (!_list_proj.empty()) {
+    bool timed = false;
+    const int parts = rowcol_parts(stream, &timed);
+    const size_t ev = 2 * (size_t)_tune_pos;
+    if (timed) hip_check(hipEventRecord(_tev[ev], stream), "hipEventRecord");
     Timing::get().begin(1, stream);
     if (_batch) {
-      op_check(gsbn_proj_rowcol_batch(_proj_dev.p, _wg_proj.p, _wg_proj_hcu.p, _nhcu_proj, _parts,
+      op_check(gsbn_proj_rowcol_batch(_proj_dev.p, _wg_proj.p, _wg_proj_hcu.p, _nhcu_proj, parts,
                                       &st, stream),
                "gsbn_proj_rowcol_batch");
     } else {
@@ -1155,9 +1220,14 @@ void ProcUpdLazy::update_gpu() {
       a.kp = a.taupdt * prn;
       a.table = a.tables[slot];
       a.table_len = a.table_lens[slot];
+      a.parts = parts;
       op_check(gsbn_proj_rowcol(&a, stream), "gsbn_proj_rowcol");
     }
     Timing::get().end(1, stream);
+    if (timed) {
+      hip_check(hipEventRecord(_tev[ev + 1], stream), "hipEventRecord");
+      if (++_tune_pos == (int)(_cand.size() * kTuneSamples)) finish_tuning();
+    }
   }
   for (auto* q : _list_proj) q->update_que(_exec);
   if (touched)

@@ -110,10 +110,12 @@ class Recall:
             pass
 
     def run(self, clamp: np.ndarray, wmask: np.ndarray, steps: int, count_from: int = 0,
-            raster: bool = False) -> Tuple[np.ndarray, np.ndarray, Optional[np.ndarray], float]:
+            raster: bool = False, time_support: bool = False
+            ) -> Tuple[np.ndarray, np.ndarray, Optional[np.ndarray], float]:
         """clamp[B, H] (clamped MCU or -1), wmask[B, H].  Returns the per-cue
-        per-HCU winner and its count, the optional raster [steps, B, H*M] and
-        the summed support-product time in ms (mfma)."""
+        per-HCU winner and its count, the optional raster [steps, B, H*M] and,
+        with time_support, the summed support-product time in ms (mfma; the
+        timing waits for every product, so it slows the run)."""
         B, H, M = self.B, self.H, self.M
         c = np.ascontiguousarray(clamp, np.int16).reshape(B, H)
         w = np.ascontiguousarray(wmask, np.float32).reshape(B, H)
@@ -122,7 +124,8 @@ class Recall:
         ras = np.zeros((steps, B, H * M), np.int8) if raster else None
         ms = ctypes.c_double(0.0)
         rc = lib().gsbn_recall_run(self._h, _p(c), _p(w), steps, count_from, _p(idx), _p(cnt),
-                                   _p(ras) if raster else None, ctypes.byref(ms))
+                                   _p(ras) if raster else None,
+                                   ctypes.byref(ms) if time_support else None)
         if rc != 0:
             raise GsbnError("gsbn_recall_run: " + lib().gsbn_last_error().decode())
         return idx, cnt, ras, ms.value

@@ -286,6 +286,11 @@ int gsbn_row_fill(gsbn_row_rec* row, int64_t n, gsbn_row_rec v, void* stream);
 int gsbn_i32_fill(int32_t* p, int64_t n, int32_t v, void* stream);
 int gsbn_syn16_fill(gsbn_syn16_rec* syn, int64_t n, gsbn_syn16_rec v, void* stream);
 
+/* device-to-device copy of `bytes` (a multiple of 16, 16-byte aligned
+ * pointers) by a plain streaming kernel; no counterpart in the reference:
+ * the calibration point of the memory-traffic counters in measurements */
+int gsbn_device_copy(void* dst, const void* src, int64_t bytes, void* stream);
+
 /* host copies of the 16-bit trace conversions (same rounding as the kernels) */
 uint16_t gsbn_host_pack_trace(int prec, float x, float scale);
 float gsbn_host_unpack_trace(int prec, uint16_t h, float inv_scale);

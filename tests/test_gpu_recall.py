@@ -142,7 +142,7 @@ def test_recall_mfma_agrees_with_exact(trained10):
     out = {}
     for mode in (R.EXACT, R.MFMA):
         rc = R.Recall.from_solver(s, spec, B, mode, seed=99)
-        idx, cnt, _, ms = rc.run(clamp, wmask, T, count_from=T // 2)
+        idx, cnt, _, ms = rc.run(clamp, wmask, T, count_from=T // 2, time_support=True)
         out[mode] = idx
         rc.close()
     ok_exact = R.recall_correct(out[R.EXACT], want)
