@@ -101,6 +101,28 @@ __device__ int block_excl_scan(int v, int* sh, int* total) {
   return res;
 }
 
+/* the same with a wave shuffle scan and two barriers (no serial section) */
+__device__ __forceinline__ int block_scan_fast(int v, int* sh, int* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+  for (int w = 0; w < nw; ++w) {
+    const int t = sh[w];
+    if (w < wid) base += t;
+    tot += t;
+  }
+  *total = tot;
+  __syncthreads();
+  return base + x - v;
+}
+
 /* epsc of unit (h,j) for incoming projection m after adding the pending row
  * contributions of the previous step, in ascending row order */
 __device__ float finalized_epsc(const gsbn_pop_args& a, const gsbn_pop_proj& p, int m, int h,
@@ -115,8 +137,21 @@ __device__ float finalized_epsc(const gsbn_pop_args& a, const gsbn_pop_proj& p, 
   return e;
 }
 
-/* queue push of source unit n spiking at exec step s: its rows become
- * active at s + di + 2; threads tid, tid+nt, ... take the unit's rows */
+/* queue push of the rows fwd_row[q0 .. q1) of a unit spiking at exec step
+ * s: they become active at s + di + 2; threads tid, tid+nt, ... take them */
+__device__ __forceinline__ void push_rows(const gsbn_fanout& f, int q0, int q1, int s, int tid,
+                                          int nt) {
+  for (int q = q0 + tid; q < q1; q += nt) {
+    const int r = f.fwd_row[q];
+    const int d = f.fwd_del[q];
+    const int hh = r / f.C;
+    const int k = r - hh * f.C;
+    const int slot = (s + d + 2) % GSBN_FUT;
+    atomicOr(f.fut + ((int64_t)slot * f.H + hh) * f.W + (k >> 5), 1u << (k & 31));
+  }
+}
+
+/* the same for source unit n */
 __device__ __forceinline__ void push_unit(const gsbn_fanout& f, int n, int s, int tid, int nt) {
   const int q1 = f.fwd_off[n + 1];
   for (int q = f.fwd_off[n] + tid; q < q1; q += nt) {
@@ -177,13 +212,20 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   float dsup_old = 0.0f, lg = 0.0f, wmv = 0.0f, e0 = 0.0f, bj0 = 0.0f, pj0 = 0.0f, ej0 = 0.0f,
         zj0 = 0.0f;
   int32_t counter_old = 0;
-  uint32_t fut0 = 0u;
+  uint32_t fut0 = 0u, fut1 = 0u, fut2 = 0u, fut3 = 0u; /* words j + i * blockDim of the ring slot */
   int pend0 = 0;
   if (live) {
     dsup_old = a.dsup[idx];
     lg = a.lginp[idx];
     wmv = a.wmask[h];
     counter_old = a.counter[idx];
+  }
+  /* fan-out range of this unit in the first outgoing projection (the queue
+   * push of step 5b) */
+  int fo0 = 0, fo1 = 0;
+  if (a.nproj_out > 0 && live) {
+    fo0 = a.outs[0].fwd_off[idx];
+    fo1 = a.outs[0].fwd_off[idx + 1];
   }
   if (nin > 0) {
     const gsbn_pop_proj& p0 = a.projs[0];
@@ -195,7 +237,33 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
       ej0 = p0.ej[idx];
       zj0 = p0.zj[idx];
     }
-    if (j < p0.W) fut0 = p0.fut[((int64_t)(a.exec % GSBN_FUT) * a.H + h) * p0.W + j];
+    const uint32_t* wslot = p0.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * p0.W;
+    const int bd = blockDim.x;
+    if (j < p0.W) fut0 = wslot[j];
+    if (j + bd < p0.W) fut1 = wslot[j + bd];
+    if (j + 2 * bd < p0.W) fut2 = wslot[j + 2 * bd];
+    if (j + 3 * bd < p0.W) fut3 = wslot[j + 3 * bd];
+  }
+  /* the draws and the first projection's postsynaptic traces do not depend
+   * on this step's results either: computed while the loads are in flight */
+  float u = 0.0f, xi = 0.0f;
+  if (live) {
+    float z;
+    gsbn_draw_unit(a.seed, (uint32_t)a.pop_id, (uint32_t)a.simstep, (uint32_t)idx, &u, &z);
+    xi = a.snoise * z;
+  }
+  float pj_n = 0.0f, ej_n = 0.0f, zj_n = 0.0f, bj_n = 0.0f;
+  bool bj_w = false;
+  if (a.do_jxx && live && nin > 0) {
+    /* update_jxx_kernel_cpu (Proj.cpp:310-363) for projection 0; the spike
+     * increment of zj follows in step 6 */
+    const gsbn_pop_proj& p0 = a.projs[0];
+    const float kp = p0.taupdt * a.prn;
+    pj_n = pj0 + (ej0 - pj0) * kp;
+    ej_n = ej0 + (zj0 - ej0) * p0.ke;
+    zj_n = zj0 * (1 - p0.kzj);
+    bj_w = kp != 0.0f;
+    if (bj_w) bj_n = p0.bgain * gsbn_logf(pj_n + p0.eps);
   }
   if (j > 0 && j < nin) sh_pend[j] = a.projs[j].pend_cnt[h];
   __syncthreads();
@@ -224,12 +292,9 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
     a.epsc[m * N + idx] = e;
   }
 
-  /* 2. random draws, support, dsup (update_sup_kernel_1_cpu, Pop.cpp:229-261) */
-  float u = 0.0f, xi = 0.0f, dsup = 0.0f;
+  /* 2. support, dsup (update_sup_kernel_1_cpu, Pop.cpp:229-261) */
+  float dsup = 0.0f;
   if (live) {
-    float z;
-    gsbn_draw_unit(a.seed, (uint32_t)a.pop_id, (uint32_t)a.simstep, (uint32_t)idx, &u, &z);
-    xi = a.snoise * z;
     float wsup = 0.0f;
     for (int m = 0; m < nin; ++m) wsup += (m == 0 ? bj0 : a.bj[m * N + idx]) + efin[m];
     float sup = a.lgbias + a.igain * lg + xi;
@@ -313,10 +378,12 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   /* 5. spiking units of this hypercolumn, ascending (.ssj, Proj.cpp:632-643) */
   int spk_tot;
   {
-    int pos = block_excl_scan(live ? spk : 0, sh_scan, &spk_tot);
+    int pos = block_scan_fast(live ? spk : 0, sh_scan, &spk_tot);
     if (live && spk) {
       a.spk_list[h * M + pos] = j;
       sh_spk[pos] = j;
+      sh_ext[pos] = fo0; /* the fan-out range of the first outgoing projection */
+      sh_f[pos] = __int_as_float(fo1);
     }
     if (j == 0) a.spk_cnt[h] = spk_tot;
   }
@@ -324,7 +391,9 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
    * step would push from si, Proj.cpp:645-664) into the outgoing rings */
   if (a.nproj_out > 0 && spk_tot > 0) {
     __syncthreads();
-    for (int o = 0; o < a.nproj_out; ++o)
+    for (int i = 0; i < spk_tot; ++i)
+      push_rows(a.outs[0], sh_ext[i], __float_as_int(sh_f[i]), a.exec, threadIdx.x, blockDim.x);
+    for (int o = 1; o < a.nproj_out; ++o)
       for (int i = 0; i < spk_tot; ++i)
         push_unit(a.outs[o], h * M + sh_spk[i], a.exec, threadIdx.x, blockDim.x);
   }
@@ -334,10 +403,17 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   if (a.do_jxx && live) {
     for (int m = 0; m < nin; ++m) {
       const gsbn_pop_proj& p = a.projs[m];
-      const float kp = p.taupdt * a.prn;
-      float pj = m == 0 ? pj0 : p.pj[idx], ej = m == 0 ? ej0 : p.ej[idx],
-            zj = m == 0 ? zj0 : p.zj[idx];
       a.epsc[m * N + idx] = efin[m] * (1 - p.kepsc);
+      if (m == 0) {
+        if (spk > 0) zj_n += p.kftj;
+        if (bj_w) a.bj[idx] = bj_n;
+        p.pj[idx] = pj_n;
+        p.ej[idx] = ej_n;
+        p.zj[idx] = zj_n;
+        continue;
+      }
+      const float kp = p.taupdt * a.prn;
+      float pj = p.pj[idx], ej = p.ej[idx], zj = p.zj[idx];
       pj += (ej - pj) * kp;
       ej += (zj - ej) * p.ke;
       zj *= (1 - p.kzj);
@@ -356,8 +432,12 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
     const int C = p.C;
     const int W = p.W;
     uint32_t* words = p.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * W;
-    for (int w = j; w < W; w += blockDim.x) {
-      const uint32_t v = (m == 0 && w == j) ? fut0 : words[w];
+    for (int w = j, i = 0; w < W; w += blockDim.x, ++i) {
+      uint32_t v;
+      if (m == 0 && i < 4)
+        v = i == 0 ? fut0 : i == 1 ? fut1 : i == 2 ? fut2 : fut3;
+      else
+        v = words[w];
       sh_bitmap[w] = v;
       if (v) words[w] = 0u;
     }
@@ -368,7 +448,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
     int c = 0;
     for (int w = w0; w < w1; ++w) c += __popc(sh_bitmap[w]);
     int tot;
-    int pos = block_excl_scan(c, sh_scan, &tot);
+    int pos = block_scan_fast(c, sh_scan, &tot);
     for (int w = w0; w < w1; ++w) {
       uint32_t bits = sh_bitmap[w];
       while (bits) {
@@ -557,10 +637,6 @@ struct RcStep {
 template <int PREC>
 __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcStep& q,
                                             const int h, const int part) {
-  __shared__ float sh_pi[RC_THREADS];
-  __shared__ int sh_a[RC_THREADS];
-  __shared__ int sh_r[RC_THREADS];
-
   const SynIO<PREC> io(a);
   const int P = q.P;
   const int M = a.M, C = a.C;
@@ -574,32 +650,34 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   const int rpt = M >= RC_THREADS ? 1 : RC_THREADS / M;
   for (int base = 0; base < mine; base += rpt) {
     const int nr = (mine - base < rpt) ? mine - base : rpt;
-    if (tid < nr) {
-      const int aa = part + P * (base + tid);
-      const int k = a.rows_list[(int64_t)h * C + aa];
-      const int r = h * C + k;
-      gsbn_row_rec st = a.rowst[r];
-      const int D = t - st.ti;
-      if (D <= 0) {
-        st.zi += a.kfti;
-        st.ti = t;
-      } else {
-        const float* c = gsbn_coef(q.table, q.table_len, D);
-        gsbn_row_catchup(c, &st.pi, &st.ei, &st.zi);
-        st.zi = st.zi + a.kfti;
-        st.ti = t;
-      }
-      a.rowst[r] = st;
-      sh_pi[tid] = st.pi;
-      sh_a[tid] = aa;
-      sh_r[tid] = r;
-    }
-    __syncthreads();
+    /* every thread of a row loads the row record with its synapse record and
+     * catches the row's i-traces up itself (all threads of the row compute
+     * the same values); the j == 0 thread writes the record back once every
+     * read of this pass is done */
+    gsbn_row_rec st0;
+    int64_t r0 = -1;
     for (int f = tid; f < nr * M; f += RC_THREADS) {
       const int rr = f / M;
       const int j = f - rr * M;
-      const int64_t sidx = (int64_t)sh_r[rr] * M + j;
+      const int aa = part + P * (base + rr);
+      const int k = a.rows_list[(int64_t)h * C + aa];
+      const int64_t r = (int64_t)h * C + k;
+      const int64_t sidx = r * M + j;
+      gsbn_row_rec st = a.rowst[r];
       SynVals s = io.load(sidx, !learn);
+      const int Dr = t - st.ti;
+      if (Dr <= 0) {
+        st.zi += a.kfti;
+        st.ti = t;
+      } else {
+        gsbn_row_catchup(gsbn_coef(q.table, q.table_len, Dr), &st.pi, &st.ei, &st.zi);
+        st.zi = st.zi + a.kfti;
+        st.ti = t;
+      }
+      if (j == 0) {
+        st0 = st;
+        r0 = r;
+      }
       float contrib;
       bool w_new = false;
       const int D = t - s.tij;
@@ -612,7 +690,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
         s.zi2 = s.zi2 + a.kfti;
         s.tij = t;
         if (learn) {
-          s.wij = gsbn_wij(s.pij, sh_pi[rr], a.pj[h * M + j], a.wgain, a.eps, a.eps2);
+          s.wij = gsbn_wij(s.pij, st.pi, a.pj[h * M + j], a.wgain, a.eps, a.eps2);
           w_new = true;
         }
         contrib = s.wij;
@@ -621,9 +699,10 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
        * update (the reference's col pass sees D == 0 here) */
       if (a.sj[h * M + j] > 0) s.zj2 += a.kftj;
       io.store(sidx, s, w_new);
-      a.contrib[((int64_t)h * C + sh_a[rr]) * M + j] = contrib;
+      a.contrib[((int64_t)h * C + aa) * M + j] = contrib;
     }
     __syncthreads();
+    if (r0 >= 0) a.rowst[r0] = st0;
   }
 
   /* slice `part` of the slots of every spiking unit of this HCU */
@@ -642,6 +721,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
       bool ok[U];
       int64_t sidx[U];
       int iiv[U], stamp[U];
+      SynVals v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = kb + u * RC_THREADS;
@@ -650,15 +730,14 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
         iiv[u] = a.ii[r];
         stamp[u] = a.act_stamp[r];
         ok[u] = k < k1;
+        /* the record is loaded with the slot checks (a skipped slot only
+         * costs an unused load) */
+        v[u] = io.load_traces(sidx[u]);
       }
-      SynVals v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         ok[u] = ok[u] & (iiv[u] >= 0) & (stamp[u] != q.exec);
-        if (ok[u])
-          v[u] = io.load_traces(sidx[u]);
-        else
-          v[u].tij = t;
+        if (!ok[u]) v[u].tij = t;
       }
       gsbn_syn_coef_t cf[U];
 #pragma unroll
