@@ -81,7 +81,8 @@ def _p(a: np.ndarray):
 
 
 class Solver:
-    """The reference's Solver over this library (Solver.cpp:10-92)."""
+    """The reference's Solver over this library (Solver.cpp:10-92); with a
+    snapshot path it resumes from that SolverState file (Solver.cpp:44-61)."""
 
     def __init__(self, net_path: str, snapshot_path: Optional[str] = None):
         L = lib()
@@ -163,6 +164,15 @@ class Solver:
         if lib().gsbn_solver_work(self._h, _p(o)) != 0:
             raise _err("work")
         return o
+
+    def snapshot(self, path: str):
+        """Write the current state as a SolverState file (what ProcSnapshot
+        writes at its steps); ``Solver(net_path, path)`` resumes from it."""
+        from . import snapshot as _snap
+        L = lib()
+        _snap._setup(L)
+        if L.gsbn_solver_snapshot(self._h, path.encode()) != 0:
+            raise _err("snapshot")
 
     def timing(self, enable: int, collect: bool = True):
         n = np.zeros(6, np.int64)

@@ -26,7 +26,8 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__",
               "-I" + os.path.join(ROCM, "include"), "-Wall", "-Wno-unused-function"] + FP_FLAGS
 
 HIP_SOURCES = ["kernels.hip", "recall.hip"]
-HOST_SOURCES = ["framework.cpp", "procedures.cpp", "proc_upd_lazy.cpp", "capi.cpp", "coeff.cpp"]
+HOST_SOURCES = ["framework.cpp", "procedures.cpp", "proc_upd_lazy.cpp", "capi.cpp", "coeff.cpp",
+                "snapshot.cpp", "structural_plasticity.cpp"]
 
 
 def _run(cmd):

@@ -48,6 +48,9 @@ class PopSpec:
     wgain: float = 1.0
     lgbias: float = 0.0
     snoise: float = 0.0
+    fanout: Optional[int] = None  # fanout_num (structural plasticity); default mcu
+    shape: Tuple[int, ...] = ()     # hypercolumn grid and offset (geometric delays)
+    position: Tuple[int, ...] = ()
 
 
 @dataclasses.dataclass
@@ -81,6 +84,22 @@ class ModeSpec:
 
 
 @dataclasses.dataclass
+class StructuralSpec:
+    """ProcStructuralPlasticity's arguments (ProcStructuralPlasitcity.cpp:17-66
+    defaults); `seed` keys its draws."""
+    period: int = 100
+    pruning_period: int = 1000
+    t_th: int = 100
+    wp: float = 0.5
+    wn: float = -0.5
+    enable_geometry: int = 0
+    enable_init_conn: int = 0
+    d_norm: float = 0.75
+    v_cond: float = 0.2
+    seed: int = 0
+
+
+@dataclasses.dataclass
 class NetSpec:
     dt: float
     pops: List[PopSpec]
@@ -103,6 +122,10 @@ class NetSpec:
     # Database API instead of a connection-map file (no ProcInitConn)
     conn_arrays: Optional[List[Tuple[np.ndarray, np.ndarray]]] = None
     trace_precision: str = "fp32"  # storage of Pij/Eij: fp32, bf16 or fp16
+    # (offset, period) of ProcSnapshot / ProcSpkRec at the end of the list
+    snapshot: Optional[Tuple[int, int]] = None
+    spike_record: Optional[Tuple[int, int]] = None
+    structural: Optional[StructuralSpec] = None  # ProcStructuralPlasticity after the exchange
 
     @property
     def total_hcu(self) -> int:
@@ -206,7 +229,8 @@ def _fmt(v: float) -> str:
 def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
              ext_spike_path: Optional[str] = None) -> str:
     """SolverParam text (gsbn.proto:4-98) with the procedure list
-    ProcExtGen, ProcUpdLazy, ProcInitConn, ProcExchangeSpike, ProcCheck."""
+    ProcExtGen, ProcUpdLazy, ProcInitConn, ProcExchangeSpike,
+    [ProcStructuralPlasticity], ProcCheck, [ProcSpkRec], [ProcSnapshot]."""
     L = []
     L.append("gen_param {")
     L.append('  stim_file: "%s"' % stim_path)
@@ -225,7 +249,11 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append("    pop_num: 1")
         L.append("    hcu_num: %d" % p.hcu)
         L.append("    mcu_num: %d" % p.mcu)
-        L.append("    fanout_num: %d" % p.mcu)
+        L.append("    fanout_num: %d" % (p.mcu if p.fanout is None else p.fanout))
+        for v in p.position:
+            L.append("    position: %d" % v)
+        for v in p.shape:
+            L.append("    shape: %d" % v)
         for f in ("taum", "wtagain", "maxfq", "igain", "wgain", "lgbias", "snoise"):
             L.append("    %s: %s" % (f, _fmt(getattr(p, f))))
         L.append("    rank: 0")
@@ -262,10 +290,26 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('proc_param { name: "ProcExchangeSpike" }')
     else:
         L.append('proc_param { name: "ProcExchangeSpike" argi { key: "direct-history" val: 0 } }')
+    sp = spec.structural
+    if sp is not None:
+        L.append("proc_param {")
+        L.append('  name: "ProcStructuralPlasticity"')
+        for key, val in (("period", sp.period), ("pruning-period", sp.pruning_period),
+                         ("t-th", sp.t_th), ("enable-geometry", sp.enable_geometry),
+                         ("enable-init-conn", sp.enable_init_conn),
+                         ("seed-lo", sp.seed & 0x7FFFFFFF), ("seed-hi", (sp.seed >> 31) & 0x7FFFFFFF)):
+            L.append('  argi { key: "%s" val: %d }' % (key, val))
+        for key, val in (("wp", sp.wp), ("wn", sp.wn), ("d-norm", sp.d_norm), ("v-cond", sp.v_cond)):
+            L.append('  argf { key: "%s" val: %s }' % (key, _fmt(val)))
+        L.append("}")
     L.append("proc_param {")
     L.append('  name: "ProcCheck"')
     L.append('  argi { key: "threashold" val: %d }' % spec.check_threshold)
     L.append("}")
+    for name, sched in (("ProcSpkRec", spec.spike_record), ("ProcSnapshot", spec.snapshot)):
+        if sched is not None:
+            L.append('proc_param { name: "%s" argi { key: "offset" val: %d } '
+                     'argi { key: "period" val: %d } }' % (name, sched[0], sched[1]))
     return "\n".join(L) + "\n"
 
 

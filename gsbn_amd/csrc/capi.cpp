@@ -24,6 +24,10 @@ struct gsbn_solver {
   gsbn::Solver* s = nullptr;
 };
 
+struct gsbn_state {
+  gsbn::SolverStateData d;
+};
+
 namespace {
 thread_local std::string g_err;
 
@@ -58,6 +62,86 @@ bool dynamic_size(const std::string& n) {
 extern "C" {
 
 const char* gsbn_last_error(void) { return g_err.c_str(); }
+
+int gsbn_solver_snapshot(gsbn_solver* s, const char* path) {
+  return guard([&] {
+    if (!s || !s->s || !path) throw Error(GSBN_ERR_ARG, "null argument");
+    gsbn::GlobalVar glv;
+    int simstep = 0;
+    float dt = 0, prn = 0;
+    glv.geti("simstep", simstep);
+    glv.getf("dt", dt);
+    glv.getf("prn", prn);
+    s->s->db().save_state(path, simstep * dt, prn);
+    return 0;
+  });
+}
+
+int gsbn_state_write(const char* path, float timestamp, float prn, int32_t nvec,
+                     const gsbn_vec_state* vecs) {
+  return guard([&] {
+    if (!path || nvec < 0 || (nvec && !vecs)) throw Error(GSBN_ERR_ARG, "null argument");
+    std::vector<gsbn::VecSpan> spans;
+    for (int32_t i = 0; i < nvec; ++i) {
+      const gsbn_vec_state& v = vecs[i];
+      if (!v.name || v.dtype < 0 || v.dtype > 6 || v.count < 0 || (v.count && !v.data))
+        throw Error(GSBN_ERR_ARG, "bad vector state " + std::to_string(i));
+      gsbn::VecSpan sp;
+      sp.name = v.name;
+      sp.type = (gsbn::DType)v.dtype;
+      sp.ld = (int)v.ld;
+      sp.count = (size_t)v.count;
+      sp.data = v.data;
+      spans.push_back(sp);
+    }
+    gsbn::write_solver_state(path, timestamp, prn, std::move(spans));
+    return 0;
+  });
+}
+
+gsbn_state* gsbn_state_read(const char* path) {
+  try {
+    if (!path) throw Error(GSBN_ERR_ARG, "null path");
+    auto* st = new gsbn_state();
+    try {
+      st->d = gsbn::read_solver_state(path);
+    } catch (...) {
+      delete st;
+      throw;
+    }
+    return st;
+  } catch (const std::exception& e) {
+    fail(e);
+    return nullptr;
+  }
+}
+
+int32_t gsbn_state_nvec(const gsbn_state* st) { return st ? (int32_t)st->d.vecs.size() : 0; }
+
+int gsbn_state_header(const gsbn_state* st, float* timestamp, float* prn) {
+  return guard([&] {
+    if (!st) throw Error(GSBN_ERR_ARG, "null state");
+    if (timestamp) *timestamp = st->d.timestamp;
+    if (prn) *prn = st->d.prn;
+    return 0;
+  });
+}
+
+int gsbn_state_vec(const gsbn_state* st, int32_t i, gsbn_vec_state* out) {
+  return guard([&] {
+    if (!st || !out) throw Error(GSBN_ERR_ARG, "null argument");
+    if (i < 0 || i >= (int32_t)st->d.vecs.size()) throw Error(GSBN_ERR_ARG, "vector index out of range");
+    const gsbn::VecState& v = st->d.vecs[(size_t)i];
+    out->name = v.name.c_str();
+    out->dtype = (int32_t)v.type;
+    out->ld = v.ld;
+    out->count = (int64_t)v.count();
+    out->data = v.bytes.empty() ? nullptr : v.bytes.data();
+    return 0;
+  });
+}
+
+void gsbn_state_free(gsbn_state* st) { delete st; }
 
 gsbn_solver* gsbn_solver_new(const char* net_path, const char* snapshot_path) {
   try {

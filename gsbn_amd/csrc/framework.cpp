@@ -7,6 +7,7 @@
 #include <sys/stat.h>
 
 #include <cctype>
+#include <cmath>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -104,6 +105,11 @@ bool GlobalVar::puts(const string key, const string val) {
 }
 
 Database::~Database() {}
+
+map<int, ProjRowOps*>& proj_row_ops() {
+  static map<int, ProjRowOps*> m;
+  return m;
+}
 
 /* ------------------------------------------------------------------ */
 /* Parser (Parser.cpp:12-43) */
@@ -209,8 +215,21 @@ Solver::Solver(type_t type, string n_path, string s_path) {
     _glv.putf("prn", 0);
     _glv.putf("old-prn", 0);
   } else {
-    (void)s_path;
-    throw Error(-5, "resuming from a snapshot is not supported by this build");
+    /* Solver.cpp:44-61: restore the vectors, then init_copy every procedure */
+    SolverStateData st = read_solver_state(s_path);
+    _database.init_copy(st);
+    st.vecs.clear();
+    _upd.init(solver_param, _database, true);
+    float dt = 0;
+    GSBN_CHECK(_glv.getf("dt", dt) && dt > 0, "dt");
+    /* the step the snapshot was taken at.  The reference truncates
+     * timestamp/dt computed in float (Solver.cpp:57), which gives one step
+     * less for about 1 in 130 steps (t = 1 s at dt = 1 ms resumes at 999);
+     * rounding to nearest is exact while the step count is below 2^23 */
+    _glv.puti("simstep", (int)std::lround((double)st.timestamp / (double)dt));
+    _glv.puti("cycle-flag", 0);
+    _glv.putf("prn", st.prn);
+    _glv.putf("old-prn", st.prn);
   }
 }
 

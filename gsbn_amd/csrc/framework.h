@@ -28,6 +28,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -249,9 +250,41 @@ class SyncVector : public VecBase {
   size_t n_ = 0, d_n_ = 0;
 };
 
+/* SolverState (gsbn.proto:100-152): one saved vector.  `bytes` holds the
+ * elements in the Database type (float for DT_F16, as VectorStateF16). */
+struct VecState {
+  string name;
+  DType type = DT_F32;
+  uint32_t ld = 1;
+  vector<uint8_t> bytes;
+  size_t count() const;
+};
+struct SolverStateData {
+  float timestamp = 0, prn = 0;
+  vector<VecState> vecs;
+};
+/* a vector to save: `count` elements of `type` at `data` (host memory) */
+struct VecSpan {
+  string name;
+  DType type = DT_F32;
+  int ld = 0;
+  size_t count = 0;
+  const void* data = nullptr;
+};
+/* snapshot.cpp: the SolverState wire encoding */
+void write_solver_state(const string& path, float timestamp, float prn, vector<VecSpan> vecs);
+SolverStateData read_solver_state(const string& path);
+
 class Database {
  public:
   ~Database();
+  /* Database::init_copy (Database.cpp:30-115): one vector per saved state */
+  void init_copy(const SolverStateData& st);
+  /* state_to_proto (Database.cpp:257-295) serialised to `path`: every
+   * vector whose name does not start with '.' */
+  void save_state(const string& path, float timestamp, float prn);
+  /* the vector was created by init_copy from a snapshot */
+  bool restored(const string& name) const { return _restored.count(name) > 0; }
   SyncVector<int8_t>* create_sync_vector_i8(const string name) { return create<int8_t>(name); }
   SyncVector<int16_t>* create_sync_vector_i16(const string name) { return create<int16_t>(name); }
   SyncVector<int32_t>* create_sync_vector_i32(const string name) { return create<int32_t>(name); }
@@ -270,7 +303,8 @@ class Database {
   }
   const map<string, std::unique_ptr<VecBase>>& all() const { return _vecs; }
 
- private:
+  /* typed create / lookup: NULL when the name exists / is missing or has
+   * another element type */
   template <typename T>
   SyncVector<T>* create(const string& name) {
     if (_vecs.count(name)) return nullptr;
@@ -284,7 +318,10 @@ class Database {
     if (it == _vecs.end() || it->second->type != dtype_of<T>::v) return nullptr;
     return static_cast<SyncVector<T>*>(it->second.get());
   }
+
+ private:
   map<string, std::unique_ptr<VecBase>> _vecs;
+  std::set<string> _restored;
 };
 
 /* ------------------------------------------------------------------ */
@@ -360,6 +397,24 @@ class Parser {
  private:
   ProcParam _proc_param;
 };
+
+/* ------------------------------------------------------------------ */
+/* Row operations on a projection's device-resident synapse state, offered
+ * by the procedure that owns the state (ProcUpdLazy) to procedures that
+ * edit connectivity (ProcStructuralPlasticity), by projection id. */
+struct ProjRowOps {
+  virtual ~ProjRowOps() {}
+  /* Prj::prune's test per row (ProcStructuralPlasticity/Prj.cpp:112-138) */
+  virtual void prune_flags(int t_min, float wp, float wn, vector<uint8_t>* flags) = 0;
+  /* Prj::remove_conn's state reset except ii/di (Prj.cpp:78-107); the rows
+   * also leave the delay queue */
+  virtual void clear_rows(const vector<int32_t>& rows) = 0;
+  /* rows were added to ii/di on the host after a step: those whose source
+   * unit spiked in that step are queued with that spike, as the next queue
+   * update would queue it (Proj.cpp:655-661) */
+  virtual void rows_added() = 0;
+};
+map<int, ProjRowOps*>& proj_row_ops();
 
 /* ------------------------------------------------------------------ */
 class ProcedureBase {

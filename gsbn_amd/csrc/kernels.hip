@@ -939,6 +939,68 @@ __global__ void row_fill_kernel(gsbn_row_rec* row, int64_t n, gsbn_row_rec v) {
     row[i] = v;
 }
 
+/* ---- structural plasticity (Prj::prune, Prj::remove_conn) ---- */
+
+/* one wave per row: the weight test is a wave vote over the row's M weights,
+ * read only for connected rows that pass the age test */
+__global__ void prune_flags_kernel(gsbn_prune_args a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= a.R) return; /* uniform per wave */
+  uint8_t f = 0;
+  if (a.ii[row] >= 0) {
+    if (a.rowst[row].ti < a.t_min) {
+      f = 1;
+    } else {
+      bool strong = false;
+      const int64_t base = row * a.M;
+      if (a.prec == GSBN_PREC_F32) {
+        for (int j = lane; j < a.M; j += 64) {
+          const float w = a.syn[base + j].wij;
+          strong = strong || w >= a.wp || w <= a.wn;
+        }
+      } else {
+        for (int j = lane; j < a.M; j += 64) {
+          const float w = a.wij[base + j];
+          strong = strong || w >= a.wp || w <= a.wn;
+        }
+      }
+      f = __ballot(strong) ? 0 : 1;
+    }
+  }
+  if (lane == 0) a.flags[row] = f;
+}
+
+static_assert(GSBN_FUT == 64, "clear_rows_kernel gives one ring slot to each lane");
+
+/* one wave per listed row; lane s clears the row's bit in ring slot s */
+__global__ void clear_rows_kernel(gsbn_clear_rows_args a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= a.n) return;
+  const int64_t r = a.rows[i];
+  if (r < 0 || r >= (int64_t)a.H * a.C) return;
+  if (lane == 0) {
+    a.rowst[r].pi = 0.0f;
+    a.rowst[r].ei = 0.0f;
+    a.rowst[r].ti = 0;
+  }
+  const int64_t base = r * a.M;
+  for (int j = lane; j < a.M; j += 64) {
+    if (a.prec == GSBN_PREC_F32) {
+      a.syn[base + j].pij = 0.0f;
+      a.syn[base + j].eij = 0.0f;
+      a.syn[base + j].zj2 = 0.0f;
+    } else {
+      a.syn16[base + j].pij = 0;
+      a.syn16[base + j].eij = 0;
+      a.syn16[base + j].zj2 = 0.0f;
+    }
+  }
+  const int h = (int)(r / a.C), k = (int)(r - (int64_t)h * a.C);
+  atomicAnd(a.fut + ((int64_t)lane * a.H + h) * a.W + (k >> 5), ~(1u << (k & 31)));
+}
+
 /* plain streaming copy, 16 bytes per lane and iteration (the reference
  * point for the memory-traffic counters) */
 __global__ void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, int64_t n16) {
@@ -1236,6 +1298,32 @@ int gsbn_row_field(gsbn_row_rec* row, int64_t n, int field, void* buf, int to_re
   if (n == 0) return 0;
   hipLaunchKernelGGL(row_field_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0,
                      (hipStream_t)stream, row, n, field, (uint32_t*)buf, to_records);
+  return launch_status();
+}
+
+int gsbn_proj_prune_flags(const gsbn_prune_args* a, void* stream) {
+  if (!a || a->R < 0 || a->M <= 0 || a->M > 1024 || a->prec < 0 || a->prec > 2) return GSBN_ERR_ARG;
+  if (a->R == 0) return 0;
+  if (!a->ii || !a->rowst || !a->flags || (a->prec == GSBN_PREC_F32 ? !a->syn : !a->wij))
+    return GSBN_ERR_ARG;
+  const int64_t blocks = (a->R + 3) / 4;
+  if (blocks > 0x7fffffff) return GSBN_ERR_ARG;
+  hipLaunchKernelGGL(prune_flags_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, *a);
+  return launch_status();
+}
+
+int gsbn_proj_clear_rows(const gsbn_clear_rows_args* a, void* stream) {
+  if (!a || a->n < 0 || a->M <= 0 || a->M > 1024 || a->H <= 0 || a->C <= 0 ||
+      a->W != (a->C + 31) / 32 || a->prec < 0 || a->prec > 2)
+    return GSBN_ERR_ARG;
+  if (a->n == 0) return 0;
+  if (!a->rows || !a->rowst || !a->fut || (a->prec == GSBN_PREC_F32 ? !a->syn : !a->syn16))
+    return GSBN_ERR_ARG;
+  const int64_t blocks = (a->n + 3) / 4;
+  if (blocks > 0x7fffffff) return GSBN_ERR_ARG;
+  hipLaunchKernelGGL(clear_rows_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, *a);
   return launch_status();
 }
 

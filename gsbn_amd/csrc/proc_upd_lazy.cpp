@@ -15,9 +15,17 @@
  *
  * Differences from the reference, by design: update_cpu() raises (this
  * build's step runs on the GPU only; there is no silent CPU fallback),
- * init_copy() raises (snapshot resume is outside this build), random draws
- * are Philox streams keyed by (seed, population, step, unit), and the
- * closed-form catch-up uses the FP64-built coefficient table.
+ * random draws are Philox streams keyed by (seed, population, step, unit),
+ * and the closed-form catch-up uses the FP64-built coefficient table.
+ *
+ * Resume (init_copy, Proj.cpp:109-207, Pop.cpp:113-218): the saved vectors
+ * are taken over instead of re-created (the reference re-creates act_/ada_,
+ * which a snapshot holds, and its CHECK fails), _eps2 is set (the reference
+ * leaves it at zero on resume), and the delay queue is rebuilt from qi_ and
+ * siq_ (the next step's active rows, saved under that name because the
+ * reference's .siq_ is not saved and its resume drops one step of row
+ * activations).  A resumed run continues bit for bit as the uninterrupted
+ * one: the draws depend only on the step.
  */
 #include <algorithm>
 #include <array>
@@ -71,7 +79,7 @@ class Proj;
 class Pop {
  public:
   void init_new(ProcParam proc_param, PopParam pop_param, Database& db, vector<Pop*>* list_pop,
-                int* hcu_cnt, int* mcu_cnt);
+                int* hcu_cnt, int* mcu_cnt, bool copy = false);
   void flush();
   gsbn_pop_args args(int simstep, int exec, float prn, int lginp_idx, int wmask_idx, bool do_jxx,
                      uint64_t seed, int wta_pop_max);
@@ -141,11 +149,16 @@ struct TableCache {
   }
 };
 
-class Proj {
+class Proj : public ProjRowOps {
  public:
+  ~Proj() override;
+  void prune_flags(int t_min, float wp, float wn, vector<uint8_t>* flags) override;
+  void clear_rows(const vector<int32_t>& rows) override;
+  void rows_added() override;
   void init_new(ProcParam proc_param, ProjParam proj_param, Database& db, vector<Proj*>* list_proj,
-                vector<Pop*>* list_pop);
+                vector<Pop*>* list_pop, bool copy = false);
   void prepare(int simstep);
+  void restore_queue(int exec);
   void update_all(int simstep, float old_prn);
   void update_jxx(float prn);
   void update_rowcol(int simstep, int exec, float prn);
@@ -166,7 +179,7 @@ class Proj {
   Pop* _ptr_dest_pop = nullptr;
   SyncVector<int32_t>*_ii = nullptr, *_qi = nullptr, *_di = nullptr, *_ssi = nullptr, *_ssj = nullptr,
                      *_ti = nullptr, *_tij = nullptr;
-  SyncVector<int8_t>*_siq = nullptr, *_si = nullptr;
+  SyncVector<int8_t>*_siq = nullptr, *_si = nullptr, *_siq_saved = nullptr;
   SyncVector<float>*_pi = nullptr, *_ei = nullptr, *_zi = nullptr, *_pj = nullptr, *_ej = nullptr,
                    *_zj = nullptr, *_pij = nullptr, *_eij = nullptr, *_zi2 = nullptr, *_zj2 = nullptr,
                    *_wij = nullptr;
@@ -191,6 +204,8 @@ class Proj {
   DevBuf<gsbn_row_rec> rowst;
   DevBuf<float> contrib;
   DevBuf<int32_t> rows_list, rows_cnt, pend_cnt, act_stamp;
+  DevBuf<uint8_t> sp_flags; /* structural plasticity scratch */
+  DevBuf<int32_t> sp_rows;
   /* delay queue: rows of each source unit (CSR) with their delays, and the
    * ring of pending activations, GSBN_FUT slots x H x W words */
   DevBuf<int32_t> fwd_off, fwd_row;
@@ -212,6 +227,11 @@ class Proj {
   uint64_t ii_version = ~0ull, di_version = ~0ull;
   int last_exec = -1;
   bool direct_que = false; /* the source population's kernel pushes this step's spikes */
+  /* resumed from a snapshot: qi_ and siq_ as saved, until the first step
+   * rebuilds the delay queue from them */
+  bool resume_pending = false;
+  vector<int32_t> resume_qi;
+  vector<int8_t> resume_siq;
   int sim_end = 0;
   std::vector<std::unique_ptr<VecView>> view_objs;
 };
@@ -353,8 +373,25 @@ void QueueView::pull(VecBase* v) {
 
 /* ---- Pop ---- */
 
+/* the vector `name`: on resume the one restored from the snapshot when the
+ * snapshot holds it, else a new one (NULL if the name is taken) */
+template <typename T>
+static SyncVector<T>* attach(Database& db, const string& name, bool copy) {
+  if (copy && db.restored(name)) return db.get<T>(name);
+  return db.create<T>(name);
+}
+
+/* on resume a restored vector must have the size this network gives it */
+static bool restored_as(Database& db, VecBase* v, size_t n, bool copy) {
+  if (!copy || !db.restored(v->name)) return false;
+  if (v->size() != n)
+    throw Error(-2, "snapshot vector " + v->name + " has " + std::to_string(v->size()) +
+                        " elements, this network needs " + std::to_string(n));
+  return true;
+}
+
 void Pop::init_new(ProcParam proc_param, PopParam pop_param, Database& db, vector<Pop*>* list_pop,
-                   int* hcu_cnt, int* mcu_cnt) {
+                   int* hcu_cnt, int* mcu_cnt, bool copy) {
   GlobalVar glv;
   _id = (int)list_pop->size();
   list_pop->push_back(this);
@@ -381,12 +418,12 @@ void Pop::init_new(ProcParam proc_param, PopParam pop_param, Database& db, vecto
   _snoise = pop_param.snoise;
   if (_rank != rank) throw Error(-5, "populations on other ranks are not supported (one process)");
   const string sid = std::to_string(_id);
-  _dsup = db.create_sync_vector_f32("dsup_" + sid);
-  _act = db.create_sync_vector_f32("act_" + sid);
-  _ada = db.create_sync_vector_f32("ada_" + sid);
-  _epsc = db.create_sync_vector_f32("epsc_" + sid);
-  _bj = db.create_sync_vector_f32("bj_" + sid);
-  _spike = db.create_sync_vector_i8("spike_" + sid);
+  _dsup = attach<float>(db, "dsup_" + sid, copy);
+  _act = attach<float>(db, "act_" + sid, copy);
+  _ada = attach<float>(db, "ada_" + sid, copy);
+  _epsc = attach<float>(db, "epsc_" + sid, copy);
+  _bj = attach<float>(db, "bj_" + sid, copy);
+  _spike = attach<int8_t>(db, "spike_" + sid, copy);
   _rnd_uniform01 = db.create_sync_vector_f32(".rnd_uniform01_" + sid);
   _rnd_normal = db.create_sync_vector_f32(".rnd_normal_" + sid);
   _wmask = db.sync_vector_f32(".wmask");
@@ -396,10 +433,10 @@ void Pop::init_new(ProcParam proc_param, PopParam pop_param, Database& db, vecto
                  _wmask && _lginp && _counter,
              "Pop vectors");
   const size_t N = (size_t)_dim_hcu * _dim_mcu;
-  _dsup->resize(N, (float)log(1.0 / _dim_mcu));
-  _act->resize(N, (float)(1.0 / _dim_mcu));
-  _ada->resize(N);
-  _spike->resize(N);
+  if (!restored_as(db, _dsup, N, copy)) _dsup->resize(N, (float)log(1.0 / _dim_mcu));
+  if (!restored_as(db, _act, N, copy)) _act->resize(N, (float)(1.0 / _dim_mcu));
+  if (!restored_as(db, _ada, N, copy)) _ada->resize(N);
+  if (!restored_as(db, _spike, N, copy)) _spike->resize(N);
   _rnd_uniform01->resize(N);
   _rnd_normal->resize(N);
   _counter->resize(N);
@@ -565,7 +602,7 @@ static int fp16_shift(float kzi, float kzj, float kfti, float kftj) {
 }
 
 void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
-                    vector<Proj*>* list_proj, vector<Pop*>* list_pop) {
+                    vector<Proj*>* list_proj, vector<Pop*>* list_pop, bool copy) {
   GlobalVar glv;
   _id = (int)list_proj->size();
   list_proj->push_back(this);
@@ -600,42 +637,49 @@ void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
   _tauepscdt = proj_param.has_tauepsc() ? dt / proj_param.tauepsc : _tauzidt;
 
   const size_t N = (size_t)_dim_hcu * _dim_mcu;
-  p->_epsc->resize((size_t)p->_dim_proj * N);
-  p->_bj->resize((size_t)p->_dim_proj * N);
+  /* the population's epsc_/bj_ hold one slice per incoming projection */
+  if (copy && db.restored(p->_epsc->name)) {
+    if (p->_epsc->size() < (size_t)p->_dim_proj * N || p->_bj->size() < (size_t)p->_dim_proj * N)
+      throw Error(-2, "snapshot vectors epsc_/bj_ are smaller than this network's projections need");
+  } else {
+    p->_epsc->resize((size_t)p->_dim_proj * N);
+    p->_bj->resize((size_t)p->_dim_proj * N);
+  }
 
   const string sid = std::to_string(_id);
-  _ii = db.create_sync_vector_i32("ii_" + sid);
-  _qi = db.create_sync_vector_i32("qi_" + sid);
-  _di = db.create_sync_vector_i32("di_" + sid);
+  _ii = attach<int32_t>(db, "ii_" + sid, copy);
+  _qi = attach<int32_t>(db, "qi_" + sid, copy);
+  _di = attach<int32_t>(db, "di_" + sid, copy);
   _ssi = db.create_sync_vector_i32(".ssi_" + sid);
   _siq = db.create_sync_vector_i8(".siq_" + sid);
-  _pi = db.create_sync_vector_f32("pi_" + sid);
-  _ei = db.create_sync_vector_f32("ei_" + sid);
-  _zi = db.create_sync_vector_f32("zi_" + sid);
-  _ti = db.create_sync_vector_i32("ti_" + sid);
+  _siq_saved = attach<int8_t>(db, "siq_" + sid, copy);
+  _pi = attach<float>(db, "pi_" + sid, copy);
+  _ei = attach<float>(db, "ei_" + sid, copy);
+  _zi = attach<float>(db, "zi_" + sid, copy);
+  _ti = attach<int32_t>(db, "ti_" + sid, copy);
   _ssj = db.create_sync_vector_i32(".ssj_" + sid);
-  _pj = db.create_sync_vector_f32("pj_" + sid);
-  _ej = db.create_sync_vector_f32("ej_" + sid);
-  _zj = db.create_sync_vector_f32("zj_" + sid);
-  _pij = db.create_sync_vector_f32("pij_" + sid);
-  _eij = db.create_sync_vector_f32("eij_" + sid);
-  _zi2 = db.create_sync_vector_f32("zi2_" + sid);
-  _zj2 = db.create_sync_vector_f32("zj2_" + sid);
-  _tij = db.create_sync_vector_i32("tij_" + sid);
-  _wij = db.create_sync_vector_f32("wij_" + sid);
-  _si = db.create_sync_vector_i8("si_" + sid);
-  GSBN_CHECK(_ii && _qi && _di && _ssi && _siq && _pi && _ei && _zi && _ti && _ssj && _pj && _ej &&
-                 _zj && _pij && _eij && _zi2 && _zj2 && _tij && _wij && _si,
+  _pj = attach<float>(db, "pj_" + sid, copy);
+  _ej = attach<float>(db, "ej_" + sid, copy);
+  _zj = attach<float>(db, "zj_" + sid, copy);
+  _pij = attach<float>(db, "pij_" + sid, copy);
+  _eij = attach<float>(db, "eij_" + sid, copy);
+  _zi2 = attach<float>(db, "zi2_" + sid, copy);
+  _zj2 = attach<float>(db, "zj2_" + sid, copy);
+  _tij = attach<int32_t>(db, "tij_" + sid, copy);
+  _wij = attach<float>(db, "wij_" + sid, copy);
+  _si = attach<int8_t>(db, "si_" + sid, copy);
+  GSBN_CHECK(_ii && _qi && _di && _ssi && _siq && _siq_saved && _pi && _ei && _zi && _ti && _ssj &&
+                 _pj && _ej && _zj && _pij && _eij && _zi2 && _zj2 && _tij && _wij && _si,
              "Proj vectors");
 
   const int64_t R = (int64_t)_dim_hcu * _dim_conn;
   const int64_t S = R * _dim_mcu;
-  _ii->resize(R, -1);
-  _di->resize(R);
-  _pj->resize(N, (float)(1.0 / _dim_mcu));
-  _ej->resize(N);
-  _zj->resize(N);
-  _si->resize(_nsrc);
+  if (!restored_as(db, _ii, (size_t)R, copy)) _ii->resize(R, -1);
+  if (!restored_as(db, _di, (size_t)R, copy)) _di->resize(R);
+  if (!restored_as(db, _pj, N, copy)) _pj->resize(N, (float)(1.0 / _dim_mcu));
+  if (!restored_as(db, _ej, N, copy)) _ej->resize(N);
+  if (!restored_as(db, _zj, N, copy)) _zj->resize(N);
+  if (!restored_as(db, _si, (size_t)_nsrc, copy)) _si->resize(_nsrc);
 
   /* device state, initial values of Proj.cpp:88-105 */
   Parser par(proc_param);
@@ -695,6 +739,7 @@ void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
   op_check(gsbn_i32_fill(act_stamp.p, R, -1, active_stream()), "fill");
   op_check(gsbn_i32_fill(reinterpret_cast<int32_t*>(fut.p), nfut, 0, active_stream()), "fill");
   _ptr_src_pop->_out.push_back(this);
+  proj_row_ops()[_id] = this;
 
   /* views over the device state */
   struct VB {
@@ -709,16 +754,32 @@ void Proj::init_new(ProcParam proc_param, ProjParam proj_param, Database& db,
       {_pi, new RowField(this, 0), (size_t)R},  {_ei, new RowField(this, 1), (size_t)R},
       {_zi, new RowField(this, 2), (size_t)R},  {_ti, new RowField(this, 3), (size_t)R},
       {_qi, new QueueView(this, 0), (size_t)R}, {_siq, new QueueView(this, 1), (size_t)R},
+      {_siq_saved, new QueueView(this, 1), (size_t)R},
       {_ssi, new QueueView(this, 2), 0},        {_ssj, new QueueView(this, 3), 0},
   };
   for (auto& e : vb) {
+    /* on resume the saved host copy is pushed into the device state before
+     * the first step (the queue: rebuilt by restore_queue) */
+    const bool keep = restored_as(db, e.v, e.n, copy);
+    if (keep && e.v == _qi) resume_qi = raw<int32_t>(e.v);
+    if (keep && e.v == _siq_saved) resume_siq = raw<int8_t>(e.v);
     view_objs.emplace_back(e.view);
     e.v->view = e.view;
     if (e.v->type == DT_F32) static_cast<SyncVector<float>*>(e.v)->set_logical_size(e.n);
     if (e.v->type == DT_I32) static_cast<SyncVector<int32_t>*>(e.v)->set_logical_size(e.n);
     if (e.v->type == DT_I8) static_cast<SyncVector<int8_t>*>(e.v)->set_logical_size(e.n);
-    e.v->status = VecBase::DEVICE_NEWER;
+    if (keep)
+      e.v->status = (e.v == _qi || e.v == _siq_saved) ? VecBase::SYNCED : VecBase::HOST_NEWER;
+    else
+      e.v->status = VecBase::DEVICE_NEWER;
     views.push_back(e.v);
+  }
+  if (copy) {
+    /* a snapshot without siq_ (written by the reference) resumes with no
+     * row active at the first step, as the reference's resume does */
+    resume_qi.resize((size_t)R, 0);
+    resume_siq.resize((size_t)R, 0);
+    resume_pending = true;
   }
   Pop* dp = _ptr_dest_pop;
   _ptr_dest_pop->_epsc->before_host_read = [dp]() { dp->flush(); };
@@ -903,6 +964,116 @@ void Proj::update_que(int exec) {
   Timing::get().end(4, active_stream());
 }
 
+/* Resume: the ring as it stood after the queue update of exec step
+ * x = exec - 1 of the uninterrupted run (the inverse of QueueView::pull):
+ * rows of siq_ active at x + 1, bit b of qi_[r] active at x + 2 + b.  When
+ * the source population pushes its own spikes, those of step x (spike_ of
+ * the snapshot) are still to be pushed, as exec x would have. */
+void Proj::restore_queue(int exec) {
+  const int H = _dim_hcu, C = _dim_conn;
+  const int64_t R = (int64_t)H * C;
+  vector<uint32_t> ring((size_t)GSBN_FUT * H * W, 0u);
+  auto set = [&](int step, int64_t r) {
+    const int h = (int)(r / C), k = (int)(r - (int64_t)h * C);
+    ring[((size_t)(step % GSBN_FUT) * H + h) * W + (k >> 5)] |= 1u << (k & 31);
+  };
+  const int* ii = _ii->cpu_data();
+  const int* di = _di->cpu_data();
+  for (int64_t r = 0; r < R; ++r) {
+    if (ii[r] < 0) continue;
+    if (resume_siq[(size_t)r]) set(exec, r);
+    for (int b = 0; b < di[r] && b < 31; ++b)
+      if ((resume_qi[(size_t)r] >> b) & 1) set(exec + 1 + b, r);
+  }
+  hip_check(hipStreamSynchronize(active_stream()), "sync");
+  if (!ring.empty())
+    hip_check(hipMemcpy(fut.p, ring.data(), ring.size() * 4, hipMemcpyHostToDevice), "H2D");
+  if (direct_que) {
+    gsbn_que_args a;
+    memset(&a, 0, sizeof(a));
+    a.Hsrc = _ptr_src_pop->_dim_hcu;
+    a.Msrc = _ptr_src_pop->_dim_mcu;
+    a.exec = exec; /* pushes as exec - 1 */
+    a.si = _ptr_src_pop->_spike->gpu_data();
+    a.out = fanout();
+    op_check(gsbn_proj_que(&a, active_stream()), "gsbn_proj_que");
+  }
+  resume_pending = false;
+  vector<int32_t>().swap(resume_qi);
+  vector<int8_t>().swap(resume_siq);
+}
+
+Proj::~Proj() {
+  auto& ops = proj_row_ops();
+  auto it = ops.find(_id);
+  if (it != ops.end() && it->second == this) ops.erase(it);
+}
+
+void Proj::prune_flags(int t_min, float wp, float wn, vector<uint8_t>* flags) {
+  const int64_t R = (int64_t)_dim_hcu * _dim_conn;
+  prepare(0); /* host writes of ii and the traces reach the device first */
+  gsbn_prune_args a;
+  memset(&a, 0, sizeof(a));
+  a.R = R;
+  a.M = _dim_mcu;
+  a.prec = prec;
+  a.ii = _ii->gpu_data();
+  a.rowst = rowst.p;
+  a.syn = syn.p;
+  a.wij = wij16.p;
+  a.t_min = t_min;
+  a.wp = wp;
+  a.wn = wn;
+  if ((int64_t)sp_flags.n < R) sp_flags.alloc((size_t)R);
+  a.flags = sp_flags.p;
+  op_check(gsbn_proj_prune_flags(&a, active_stream()), "gsbn_proj_prune_flags");
+  flags->resize((size_t)R);
+  sp_flags.download(flags->data(), (size_t)R);
+}
+
+void Proj::clear_rows(const vector<int32_t>& rows) {
+  if (rows.empty()) return;
+  const int64_t R = (int64_t)_dim_hcu * _dim_conn;
+  for (int32_t r : rows)
+    if (r < 0 || r >= R) throw Error(-4, "clear_rows: row out of range");
+  prepare(0);
+  sp_rows.upload(rows.data(), rows.size());
+  gsbn_clear_rows_args a;
+  memset(&a, 0, sizeof(a));
+  a.rows = sp_rows.p;
+  a.n = (int64_t)rows.size();
+  a.M = _dim_mcu;
+  a.prec = prec;
+  a.H = _dim_hcu;
+  a.C = _dim_conn;
+  a.W = W;
+  a.rowst = rowst.p;
+  a.syn = syn.p;
+  a.syn16 = syn16.p;
+  a.fut = fut.p;
+  op_check(gsbn_proj_clear_rows(&a, active_stream()), "gsbn_proj_clear_rows");
+  /* host copies of the traces and the queue views are stale now */
+  mark_views_device();
+}
+
+void Proj::rows_added() {
+  /* the fan-out lists follow ii/di now */
+  prepare(0);
+  /* with the population kernel doing the queue push, the spikes of the last
+   * step went out over the old fan-out lists; pushing them again over the
+   * new ones adds the new rows' bits (setting a bit twice changes nothing) */
+  if (!direct_que || last_exec < 0) return;
+  gsbn_que_args a;
+  memset(&a, 0, sizeof(a));
+  a.Hsrc = _ptr_src_pop->_dim_hcu;
+  a.Msrc = _ptr_src_pop->_dim_mcu;
+  a.exec = last_exec + 1; /* pushes as exec - 1 */
+  a.si = _ptr_src_pop->_spike->gpu_data();
+  a.out = fanout();
+  op_check(gsbn_proj_que(&a, active_stream()), "gsbn_proj_que");
+  mark_views_device();
+}
+
 void Proj::mark_views_device() {
   for (VecBase* v : views) v->status = VecBase::DEVICE_NEWER;
 }
@@ -913,14 +1084,15 @@ class ProcUpdLazy : public ProcedureBase {
   REGISTER(ProcUpdLazy)
  public:
   ~ProcUpdLazy() override;
-  void init_new(SolverParam solver_param, Database& db) override;
-  void init_copy(SolverParam solver_param, Database& db) override;
+  void init_new(SolverParam solver_param, Database& db) override { init(solver_param, db, false); }
+  void init_copy(SolverParam solver_param, Database& db) override { init(solver_param, db, true); }
   void update_cpu() override;
   void update_gpu() override;
   int64_t work(int i);
   int64_t sweep_synapses = 0, sweeps = 0, execs = 0;
 
  private:
+  void init(SolverParam solver_param, Database& db, bool copy);
   int kp_slot(float prn);
   void refresh_args(int simstep);
   void build_batch_maps();
@@ -1008,7 +1180,8 @@ void ProcUpdLazy::finish_tuning() {
   _glv.puti("rowcol-parts", _parts);
 }
 
-void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
+/* init_new (ProcUpdLazy.cpp:8-46) and init_copy (ProcUpdLazy.cpp:48-86) */
+void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   NetParam net_param = solver_param.net_param;
   ProcParam proc_param = get_proc_param(solver_param);
   Parser par(proc_param);
@@ -1025,7 +1198,7 @@ void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
     for (uint32_t j = 0; j < pp.pop_num; j++) {
       Pop* p = new Pop();
       try {
-        p->init_new(proc_param, pp, db, &_list_pop, &hcu_cnt, &mcu_cnt);
+        p->init_new(proc_param, pp, db, &_list_pop, &hcu_cnt, &mcu_cnt, copy);
       } catch (...) {
         if (_list_pop.empty() || _list_pop.back() != p) delete p;
         throw;
@@ -1041,7 +1214,7 @@ void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
       Proj* proj = new Proj();
       proj->tcache = &_tables;
       try {
-        proj->init_new(proc_param, jp, db, &_list_proj, &_list_pop);
+        proj->init_new(proc_param, jp, db, &_list_proj, &_list_pop, copy);
       } catch (...) {
         if (_list_proj.empty() || _list_proj.back() != proj) delete proj;
         throw;
@@ -1050,10 +1223,6 @@ void ProcUpdLazy::init_new(SolverParam solver_param, Database& db) {
   }
   hip_check(hipStreamSynchronize(active_stream()), "sync");
   _glv.puti("updlazy-ready", 1);
-}
-
-void ProcUpdLazy::init_copy(SolverParam, Database&) {
-  throw Error(-5, "ProcUpdLazy::init_copy: snapshot resume is not supported by this build");
 }
 
 void ProcUpdLazy::update_cpu() {
@@ -1155,6 +1324,8 @@ void ProcUpdLazy::update_gpu() {
   const bool touched = VecBase::host_epoch() != _epoch_seen;
   if (touched)
     for (auto* q : _list_proj) q->prepare(simstep);
+  for (auto* q : _list_proj)
+    if (q->resume_pending) q->restore_queue(_exec);
   if (touched || _args_dirty || _tables.version != _tables_seen) refresh_args(simstep);
 
   /* current .lginp / .wmask rows (Pop.cpp:331-332) */

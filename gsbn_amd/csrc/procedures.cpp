@@ -11,7 +11,14 @@
  *   ProcCheck          recall scoring (ProcCheck.cpp:8-237,
  *                      ProcCheck/Pop.cpp:35-60); counting and the per-HCU
  *                      argmax run on the device
+ *   ProcSnapshot       SolverState file of every saved vector each `period`
+ *                      steps from `offset` (ProcSnapshot.cpp:8-80)
+ *   ProcSpkRec         spike raster and active-row/unit CSV files
+ *                      (ProcSpkRec.cpp:10-210, ProcSpkRec/Pop.cpp:30-52,
+ *                      ProcSpkRec/Prj.cpp:95-137)
  */
+#include <sys/stat.h>
+
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -163,7 +170,14 @@ class ProcInitConn : public ProcedureBase {
   REGISTER(ProcInitConn)
  public:
   void init_new(SolverParam solver_param, Database& db) override;
-  void init_copy(SolverParam solver_param, Database& db) override { init_new(solver_param, db); }
+  /* ProcInitConn.cpp:79-81 re-runs init_new, which adds the map's rows again
+   * to the free slots of the restored ii_ (duplicating every row of a
+   * sparse map); a snapshot's ii_/di_ are kept as saved instead */
+  void init_copy(SolverParam solver_param, Database& db) override {
+    for (size_t i = 0; i < solver_param.net_param.proj_param.size(); ++i)
+      if (db.restored("ii_" + std::to_string(i))) return;
+    init_new(solver_param, db);
+  }
   void update_cpu() override {}
   void update_gpu() override {}
 };
@@ -314,6 +328,173 @@ void ProcExchangeSpike::update_gpu() {
     if (l.sj->size())
       hip_check(hipMemcpyAsync(dst, src, l.sj->size(), hipMemcpyDeviceToDevice, active_stream()),
                 "exchange copy");
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* the directory <log-dir><procedure name>, created when missing */
+static string record_dir(GlobalVar& glv, const string& proc) {
+  string log_dir;
+  GSBN_CHECK(glv.gets("log-dir", log_dir) && !log_dir.empty(), "log-dir");
+  const string dir = log_dir + proc;
+  struct stat info;
+  if (!(stat(dir.c_str(), &info) == 0 && (info.st_mode & S_IFDIR))) {
+    const string cmd = "mkdir -p '" + dir + "'";
+    if (system(cmd.c_str()) != 0) throw Error(-2, "Cannot create directory for state records!");
+  }
+  return dir;
+}
+
+/* offset (>= 0, default 0) and period (> 0, default 1) of a recorder */
+static void record_schedule(Parser& par, int* offset, int* period) {
+  if (!par.argi("offset", *offset)) *offset = 0;
+  GSBN_CHECK(*offset >= 0, "offset >= 0");
+  if (!par.argi("period", *period)) *period = 1;
+  GSBN_CHECK(*period > 0, "period > 0");
+}
+
+class ProcSnapshot : public ProcedureBase {
+  REGISTER(ProcSnapshot)
+ public:
+  void init_new(SolverParam solver_param, Database& db) override;
+  void init_copy(SolverParam solver_param, Database& db) override { init_new(solver_param, db); }
+  void update_cpu() override { update_gpu(); }
+  void update_gpu() override;
+
+ private:
+  GlobalVar _glv;
+  Database* _db = nullptr;
+  string _directory;
+  int _offset = 0, _period = 1;
+};
+REGISTERIMPL(ProcSnapshot);
+
+void ProcSnapshot::init_new(SolverParam solver_param, Database& db) {
+  _db = &db;
+  _directory = record_dir(_glv, proc_name());
+  Parser par(get_proc_param(solver_param));
+  record_schedule(par, &_offset, &_period);
+}
+
+void ProcSnapshot::update_gpu() {
+  int cycle_flag = 0, simstep = 0;
+  float dt = 0, prn = 0;
+  GSBN_CHECK(_glv.geti("cycle-flag", cycle_flag), "cycle-flag");
+  if (cycle_flag != 1) return;
+  GSBN_CHECK(_glv.geti("simstep", simstep), "simstep");
+  GSBN_CHECK(_glv.getf("dt", dt), "dt");
+  GSBN_CHECK(_glv.getf("prn", prn), "prn");
+  GSBN_CHECK(simstep >= 0 && dt >= 0, "simstep, dt");
+  if (simstep < _offset || simstep % _period != 0) return;
+  const float t = simstep * dt;
+  _db->save_state(_directory + "/Snapshot_" + std::to_string(t) + ".bin", t, prn);
+}
+
+/* ------------------------------------------------------------------ */
+class ProcSpkRec : public ProcedureBase {
+  REGISTER(ProcSpkRec)
+ public:
+  void init_new(SolverParam solver_param, Database& db) override;
+  void init_copy(SolverParam solver_param, Database& db) override { init_new(solver_param, db); }
+  void update_cpu() override { update_gpu(); }
+  void update_gpu() override;
+
+ private:
+  struct PopRec {
+    int id, H, M;
+    SyncVector<int8_t>* spike;
+    int buffer = 1; /* spike_ rows kept (ld > 0), the row of step t is t % buffer */
+  };
+  struct PrjRec {
+    int id;
+    SyncVector<int32_t>* ssi;
+    SyncVector<int32_t>* ssj;
+  };
+  GlobalVar _glv;
+  string _directory;
+  int _offset = 0, _period = 1;
+  vector<PopRec> _pops;
+  vector<PrjRec> _prjs;
+};
+REGISTERIMPL(ProcSpkRec);
+
+void ProcSpkRec::init_new(SolverParam solver_param, Database& db) {
+  _directory = record_dir(_glv, proc_name());
+  Parser par(get_proc_param(solver_param));
+  record_schedule(par, &_offset, &_period);
+  float dt = 0;
+  GSBN_CHECK(_glv.getf("dt", dt), "dt");
+  int id = 0;
+  for (auto& pp : solver_param.net_param.pop_param) {
+    for (uint32_t k = 0; k < pp.pop_num; ++k, ++id) {
+      PopRec r;
+      r.id = id;
+      r.H = (int)pp.hcu_num;
+      r.M = (int)pp.mcu_num;
+      r.spike = db.sync_vector_i8("spike_" + std::to_string(id));
+      GSBN_CHECK(r.spike, "spike_" + std::to_string(id) + " (ProcUpdLazy must come first)");
+      if (r.spike->ld() > 0) r.buffer = (int)(r.spike->size() / r.spike->ld());
+      _pops.push_back(r);
+      std::fstream out(_directory + "/spk_pop_" + std::to_string(id) + ".csv",
+                       std::ios::out | std::ofstream::trunc);
+      out << r.H << "," << r.M << "," << dt << "," << pp.maxfq << std::endl;
+    }
+  }
+  for (size_t i = 0; i < solver_param.net_param.proj_param.size(); ++i) {
+    PrjRec r;
+    r.id = (int)i;
+    r.ssi = db.sync_vector_i32(".ssi_" + std::to_string(i));
+    r.ssj = db.sync_vector_i32(".ssj_" + std::to_string(i));
+    GSBN_CHECK(r.ssi && r.ssj, ".ssi_/.ssj_ of projection " + std::to_string(i));
+    _prjs.push_back(r);
+    for (const char* f : {"/ssi_pop_", "/ssj_pop_"})
+      std::fstream(_directory + f + std::to_string(i) + ".csv", std::ios::out | std::ofstream::trunc);
+  }
+}
+
+void ProcSpkRec::update_gpu() {
+  int cycle_flag = 0, simstep = 0;
+  GSBN_CHECK(_glv.geti("cycle-flag", cycle_flag), "cycle-flag");
+  if (cycle_flag != 1) return;
+  GSBN_CHECK(_glv.geti("simstep", simstep), "simstep");
+  GSBN_CHECK(simstep >= 0, "simstep >= 0");
+  if (simstep < _offset || simstep % _period != 0) return;
+  for (auto& p : _pops) {
+    /* Pop::record (ProcSpkRec/Pop.cpp:30-52): "step,unit,unit,..." when
+     * the step has spikes */
+    const int row = p.buffer > 1 ? simstep % p.buffer : 0;
+    const int8_t* spk = p.spike->cpu_data() + (size_t)row * p.H * p.M;
+    std::ostringstream line;
+    bool any = false;
+    for (int i = 0; i < p.H * p.M; i++)
+      if (spk[i] > 0) {
+        if (!any) line << simstep;
+        any = true;
+        line << "," << i;
+      }
+    if (any) {
+      std::fstream out(_directory + "/spk_pop_" + std::to_string(p.id) + ".csv",
+                       std::ios::out | std::ios::app);
+      out << line.str() << std::endl;
+    }
+  }
+  /* Prj::record_ssj / record_ssi (ProcSpkRec/Prj.cpp:95-137): "step,1"
+   * when unit / row 9 is in the step's active list, else "step,0" */
+  for (auto& r : _prjs) {
+    const std::pair<const char*, SyncVector<int32_t>*> files[2] = {{"/ssj_pop_", r.ssj},
+                                                                   {"/ssi_pop_", r.ssi}};
+    for (auto& f : files) {
+      const int32_t* d = f.second->cpu_data();
+      bool active = false;
+      for (size_t i = 0; i < f.second->size(); i++)
+        if (d[i] == 9) {
+          active = true;
+          break;
+        }
+      std::fstream out(_directory + f.first + std::to_string(r.id) + ".csv",
+                       std::ios::out | std::ios::app);
+      out << simstep << "," << (active ? 1 : 0) << std::endl;
+    }
   }
 }
 

@@ -268,6 +268,43 @@ int gsbn_check_count_multi(const gsbn_check_item* items_dev, int32_t nitems, voi
 int gsbn_check_argmax_multi(const gsbn_check_item* items_dev, int32_t nitems, int32_t* out_idx,
                             int32_t* out_cnt, void* stream);
 
+/* Structural plasticity, device half (ProcStructuralPlasitcity.cpp:280-300,
+ * ProcStructuralPlasticity/Prj.cpp:74-138).
+ *
+ * gsbn_proj_prune_flags: Prj::prune's test for every row r < R:
+ * flags[r] = 1 when ii[r] >= 0 and either ti[r] < t_min or no weight of the
+ * row is >= wp or <= wn; else 0.  wij from the FP32 records (prec
+ * GSBN_PREC_F32) or from the float weight array of the 16-bit forms. */
+typedef struct {
+  int64_t R;
+  int32_t M, prec;
+  const int32_t* ii;
+  const gsbn_row_rec* rowst;
+  const gsbn_syn_rec* syn;
+  const float* wij;
+  int32_t t_min;
+  float wp, wn;
+  int32_t pad;
+  uint8_t* flags;
+} gsbn_prune_args;
+int gsbn_proj_prune_flags(const gsbn_prune_args* a, void* stream);
+
+/* gsbn_proj_clear_rows: Prj::remove_conn's state reset for rows[0..n)
+ * (device pointer; entries outside [0, H*C) are skipped): pi, ei, ti = 0 and
+ * pij, eij, zj2 of the row's M synapses = 0 (zi, zi2, wij, tij kept), and
+ * the row's bit cleared in every slot of the pending-activation ring.  ii/di
+ * are the caller's (host vectors). */
+typedef struct {
+  const int32_t* rows;
+  int64_t n;
+  int32_t M, prec, H, C, W, pad;
+  gsbn_row_rec* rowst;
+  gsbn_syn_rec* syn;
+  gsbn_syn16_rec* syn16;
+  uint32_t* fut;
+} gsbn_clear_rows_args;
+int gsbn_proj_clear_rows(const gsbn_clear_rows_args* a, void* stream);
+
 /* field gather/scatter between the record arrays and plain vectors
  * field: 0 pij 1 eij 2 zi2 3 zj2 4 wij 5 tij (syn); 0 pi 1 ei 2 zi 3 ti (row) */
 int gsbn_syn_field(gsbn_syn_rec* syn, int64_t n, int field, void* buf, int to_records,
@@ -360,8 +397,9 @@ int gsbn_spike_gemm(int32_t B, int32_t N, int32_t Kp, const uint16_t* S, const u
 /* ---- solver level ---- */
 typedef struct gsbn_solver gsbn_solver;
 
-/* Solver(n_path, s_path) (Solver.cpp:10-65); s_path must be NULL or empty
- * (snapshot resume is outside this build). NULL on error, message in
+/* Solver(n_path, s_path) (Solver.cpp:10-65): a new network when s_path is
+ * NULL or empty, else resumed from the SolverState file s_path
+ * (Solver.cpp:44-61, Database.cpp:30-115).  NULL on error, message in
  * gsbn_last_error(). */
 gsbn_solver* gsbn_solver_new(const char* net_path, const char* snapshot_path);
 void gsbn_solver_free(gsbn_solver* s);
@@ -388,6 +426,34 @@ int gsbn_solver_work(gsbn_solver* s, int64_t* out5);
  * check), 0 stops timing, < 0 also clears the totals; when launches6/ms6 are
  * given the totals {launches, total_ms} per kind are written first. */
 int gsbn_solver_timing(gsbn_solver* s, int enable, int64_t* launches6, double* ms6);
+
+/* write the solver's state now, as ProcSnapshot does at its steps
+ * (ProcSnapshot.cpp:69-78): timestamp simstep*dt, the current prn */
+int gsbn_solver_snapshot(gsbn_solver* s, const char* path);
+
+/* ---- SolverState files (gsbn.proto:100-152), no device needed ----
+ * One saved vector: `count` elements of type `dtype` (codes as above; 4 f16
+ * is held as float) at `data`, with the leading dimension `ld`. */
+typedef struct gsbn_vec_state {
+  const char* name;
+  int32_t dtype;
+  uint32_t ld;
+  int64_t count;
+  const void* data;
+} gsbn_vec_state;
+/* SolverState::SerializeToOstream of {timestamp, prn, vectors}
+ * (ProcSnapshot.cpp:69-78, Database.cpp:257-295): vectors of one type in
+ * name order, types in field order, whatever the order of `vecs` */
+int gsbn_state_write(const char* path, float timestamp, float prn, int32_t nvec,
+                     const gsbn_vec_state* vecs);
+/* SolverState::ParseFromIstream (Solver.cpp:45-51); NULL on error */
+typedef struct gsbn_state gsbn_state;
+gsbn_state* gsbn_state_read(const char* path);
+int32_t gsbn_state_nvec(const gsbn_state* st);
+int gsbn_state_header(const gsbn_state* st, float* timestamp, float* prn);
+/* vector i in file order; name and data point into `st` */
+int gsbn_state_vec(const gsbn_state* st, int32_t i, gsbn_vec_state* out);
+void gsbn_state_free(gsbn_state* st);
 
 /* parse a SolverParam prototxt (and its stimulus file) without a device:
  * out4 = {populations, projections, mode_params, proc_params} */

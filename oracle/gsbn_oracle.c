@@ -67,6 +67,17 @@ void gor_draw(uint64_t seed, uint32_t pop, uint32_t step, uint32_t unit, float* 
   *z = (float)(rad * gsbn_cos2pi(u2));
 }
 
+/* the same mapping for any counter (structural plasticity's draws) */
+void gor_draw_ctr(const uint32_t ctr[4], const uint32_t key[2], float* u, float* z) {
+  uint32_t w[4];
+  gor_philox4x32_10(ctr, key, w);
+  *u = (float)(w[0] >> 8) * 5.9604644775390625e-08f;
+  double u1 = (double)((w[1] >> 8) + 1u) * 5.9604644775390625e-08;
+  double u2 = (double)(w[2] >> 8) * 5.9604644775390625e-08;
+  double rad = gsbn_sqrt(-2.0 * gsbn_log(u1));
+  *z = (float)(rad * gsbn_cos2pi(u2));
+}
+
 /* ------------------------------------------------------------------ */
 /* coefficient table entry: divided differences of x -> exp(-D x) from the
  * 3x3 bidiagonal matrix exponential, scaling and squaring (McCurdy, Ng,

@@ -107,6 +107,8 @@ void gor_ref32_syn(float kp, float ke, float kzi, float kzj, int pdt, float* pij
 /* independent Philox4x32-10 and the per-unit draw */
 void gor_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void gor_draw(uint64_t seed, uint32_t pop, uint32_t step, uint32_t unit, float* u, float* z);
+/* the same uniform / normal mapping for an arbitrary counter */
+void gor_draw_ctr(const uint32_t ctr[4], const uint32_t key[2], float* u, float* z);
 
 /* the deterministic exp/log of detmath.h, for host-side constants */
 float gor_det_logf(float x);

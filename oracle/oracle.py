@@ -13,6 +13,9 @@ procedures around it:
 * ``Check``: ProcCheck (ProcCheck.cpp:8-143 init, :149-237 update;
   ProcCheck/Pop.cpp:35-60): per-window spike counts for prn==0 modes, first
   strict argmax per HCU, the correctness criterion of ProcCheck.cpp:196.
+* ``StructuralPlasticity``: ProcStructuralPlasticity
+  (ProcStructuralPlasitcity.cpp:8-608, ProcStructuralPlasticity/Pop.cpp,
+  Prj.cpp) with this build's seeded draws; see the class for the pins.
 """
 
 from __future__ import annotations
@@ -80,6 +83,7 @@ def lib():
         L.gor_ref32_row.argtypes = [f32, f32, f32, i32, vp, vp, vp]
         L.gor_ref32_syn.argtypes = [f32, f32, f32, f32, i32, vp, vp, vp, vp]
         L.gor_philox4x32_10.argtypes = [vp, vp, vp]
+        L.gor_draw_ctr.argtypes = [vp, vp, vp, vp]
         L.gor_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.c_uint32, vp, vp]
         L.gor_work_counters.argtypes = [vp, vp]
@@ -193,7 +197,8 @@ class ExtGen:
                 if e >= end:
                     e = end
                 self.modes.append(dict(begin=begin, end=e, lgidx=lg, lgexp=lx, wmask=wm,
-                                       prn=float(np.float32(m.prn))))
+                                       prn=float(np.float32(m.prn)),
+                                       plasticity=bool(getattr(m, "plasticity", 1))))
                 lg += m.lgidx_step
                 lx += m.lgexp_step
                 wm += m.wmask_step
@@ -210,6 +215,7 @@ class ExtGen:
         self.cycle_flag = 0
         self.wmask_idx = 0
         self.lginp_idx = 0
+        self.plasticity = False
         self.l_on = det_logf(np.float32(1) + self.eps)
         self.l_off = det_logf(np.float32(0) + self.eps)
 
@@ -222,6 +228,7 @@ class ExtGen:
                 self.old_prn = self.prn
                 self.prn = m["prn"]
                 self.wmask_idx = m["wmask"]
+                self.plasticity = m["plasticity"]
                 self.lginp_idx = 0
                 self.cycle_flag = 1
                 if self.old_lgidx != m["lgidx"]:
@@ -309,6 +316,253 @@ class Check:
         self.cursor += 1
 
 
+class SpDraws:
+    """The structural-plasticity draws of one (step, lane): Philox4x32-10
+    blocks, key = seed, counter = (block, step, lane, 1), consumed word by
+    word."""
+
+    def __init__(self, seed: int, step: int, lane: int):
+        self.key = np.array([seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF], np.uint32)
+        self.step, self.lane, self.block = step & 0xFFFFFFFF, lane, 0
+        self.words: List[int] = []
+
+    def _ctr(self):
+        c = np.array([self.block, self.step, self.lane, 1], np.uint32)
+        self.block += 1
+        return c
+
+    def word(self) -> int:
+        if not self.words:
+            self.words = [int(w) for w in philox(self._ctr(), self.key)]
+        return self.words.pop(0)
+
+    def uniform(self) -> np.float32:
+        return np.float32(self.word() >> 8) * np.float32(2.0 ** -24)
+
+    def below(self, n: int) -> int:
+        return (self.word() * n) >> 32
+
+    def shuffle(self, v: list):
+        for i in range(len(v), 1, -1):
+            j = self.below(i)
+            v[i - 1], v[j] = v[j], v[i - 1]
+
+    def normal(self, m: np.float32, s: np.float32) -> np.float32:
+        self.words = []
+        u = np.zeros(1, np.float32)
+        z = np.zeros(1, np.float32)
+        lib().gor_draw_ctr(_p(self._ctr()), _p(self.key), _p(u), _p(z))
+        return np.float32(np.float32(m) + np.float32(np.float32(s) * z[0]))
+
+
+class StructuralPlasticity:
+    """ProcStructuralPlasticity restated for the oracle.
+
+    Follows ProcStructuralPlasitcity.cpp:8-608 and the helper classes of
+    ProcStructuralPlasticity/Pop.cpp:5-95 and Prj.cpp:5-138 on one rank (the
+    MPI window exchange becomes plain array access), with this build's
+    choices where the reference is not reproducible or is wrong:
+    the draws of ``SpDraws`` instead of random_device-seeded std::mt19937 and
+    the Random class; a hypercolumn's own slots decide whether it has a free
+    slot (Prj.cpp:100 looks at hypercolumn 0's); a removed row leaves the
+    delay queue (qi = siq = 0; the reference's update_que_cpu skips rows with
+    ii < 0, Proj.cpp:651-653, freezing them); geometric delays are clamped to
+    [1, 31].
+
+    Pinned by tests/test_oracle_structural.py: brute-force hand cases of the
+    prune test and the state reset, the fan-out bookkeeping, the growth
+    order, and invariants (no duplicate source in a hypercolumn, fan-out
+    never below zero for grown rows, counts conserved).
+    """
+
+    LANE_GROW, LANE_DELAY, LANE_INIT, LANE_INIT_DELAY = 0, 1, 2, 3
+
+    def __init__(self, spec, net: "OracleNet"):
+        sp = spec.structural
+        self.net = net
+        self.dt = np.float32(spec.dt)
+        self.period = sp.period if sp.period > 0 else 100
+        self.pruning_period = sp.pruning_period if sp.pruning_period > 0 else 1000
+        self.t_th = sp.t_th if sp.t_th > 0 else 100
+        self.wp, self.wn = np.float32(sp.wp), np.float32(sp.wn)
+        self.d_norm = np.float32(max(sp.d_norm, 0.0))
+        self.v_cond = np.float32(sp.v_cond if sp.v_cond > 0 else 1.0)
+        self.geometry = 1 if sp.enable_geometry else 0
+        self.seed = sp.seed
+        self.pops = []
+        for i, p in enumerate(spec.pops):
+            fan = p.mcu if p.fanout is None else p.fanout
+            self.pops.append(dict(id=i, H=p.hcu, M=p.mcu, shape=list(p.shape),
+                                  position=list(p.position),
+                                  fanout=np.full(p.hcu * p.mcu, fan, np.int64), prjs=[]))
+        self.prjs = []
+        for i, q in enumerate(spec.projs):
+            sp_, dp = spec.pops[q.src], spec.pops[q.dest]
+            C = sp_.hcu * sp_.mcu
+            if C > q.slot_num:
+                C = int(q.slot_num)
+            self.prjs.append(dict(id=i, src=q.src, dest=q.dest, H=dp.hcu, M=dp.mcu, C=C))
+            self.pops[q.src]["prjs"].append(i)
+        if sp.enable_init_conn:
+            self.grow(0, self.LANE_INIT, self.LANE_INIT_DELAY, -1, np.float32(-1.0))
+
+    def update(self, simstep: int, cycle_flag: int, plasticity: bool):
+        """ProcStructuralPlasitcity.cpp:260-290."""
+        if simstep != 0 and cycle_flag != 1:
+            return
+        if not plasticity and simstep != 0:
+            return
+        if simstep % self.pruning_period == 0:
+            self.prune(simstep)
+        if simstep % self.period == 0:
+            thr = int(0.2 * self.period * 100 * float(self.dt))
+            self.grow(simstep, self.LANE_GROW, self.LANE_DELAY, thr, np.float32(0.5))
+
+    def prune(self, simstep: int):
+        """STAGE 1-2 (ProcStructuralPlasitcity.cpp:292-352); Prj::prune and
+        Prj::remove_conn (Prj.cpp:78-138)."""
+        assert self.wp > self.wn, "w+ should be larger than w- !!"
+        net = self.net
+        for q in self.prjs:
+            s = "_%d" % q["id"]
+            ii, di, ti = net.vector("ii" + s), net.vector("di" + s), net.vector("ti" + s)
+            wij = net.vector("wij" + s).reshape(-1, q["M"])
+            t_min = simstep - self.t_th
+            removed = []
+            for r in range(ii.size):
+                if ii[r] < 0:
+                    continue
+                if ti[r] < t_min:
+                    removed.append(r)
+                    continue
+                if not np.any((wij[r] >= self.wp) | (wij[r] <= self.wn)):
+                    removed.append(r)
+            if not removed:
+                continue
+            fan = self.pops[q["src"]]["fanout"]
+            for r in removed:
+                fan[ii[r]] += 1
+            rows = np.array(removed, np.int64)
+            ii[rows] = -1
+            di[rows] = 0
+            ti[rows] = 0
+            net.write("ii" + s, ii)
+            net.write("di" + s, di)
+            net.write("ti" + s, ti)
+            for name in ("pi", "ei", "qi", ".siq"):
+                v = net.vector(name + s)
+                v[rows] = 0
+                net.write(name + s, v)
+            for name in ("pij", "eij", "zj2"):
+                v = net.vector(name + s).reshape(-1, q["M"])
+                v[rows] = 0
+                net.write(name + s, v.ravel())
+
+    def _hcu_coor(self, pop, hcu: int):
+        """Pop::hcu_coor (ProcStructuralPlasticity/Pop.cpp:65-93)."""
+        coor = []
+        index = hcu
+        for d in pop["shape"]:
+            coor.append(index % d)
+            index //= d
+        for i, p in enumerate(pop["position"]):
+            if i < len(coor):
+                coor[i] += p
+            else:
+                coor.append(p)
+        return coor
+
+    def _delay(self, draws: SpDraws, q, src_unit: int, dest_hcu: int) -> int:
+        """delay_cycle (ProcStructuralPlasitcity.cpp:578-608), clamped to [1, 31]."""
+        p0, p1 = self.pops[q["src"]], self.pops[q["dest"]]
+        a = self._hcu_coor(p0, src_unit // p0["M"])
+        b = self._hcu_coor(p1, dest_hcu)
+        n = max(len(a), len(b))
+        a += [0] * (n - len(a))
+        b += [0] * (n - len(b))
+        d2 = np.float32(0)
+        for x, y in zip(a, b):
+            d2 = np.float32(d2 + np.float32((x - y) * (x - y)))
+        dist = np.float32(np.sqrt(d2))
+        mean = np.float32(np.float32(np.float32(self.d_norm * dist) / self.v_cond) + np.float32(1))
+        tij = draws.normal(mean, np.float32(np.float32(0.1) * mean))
+        delay = int(np.ceil(float(tij) * 0.001 / float(self.dt)))
+        return min(31, max(1, delay))
+
+    def grow(self, step: int, lane: int, delay_lane: int, mcu_thr: int, hcu_thr: np.float32):
+        """STAGE 3-5 (ProcStructuralPlasitcity.cpp:354-576), init_conn
+        (:109-258); Pop::get_avail_active_mcu_list (Pop.cpp:52-63),
+        Prj::get_avail_active_hcu_list (Prj.cpp:84-110)."""
+        net = self.net
+        local, avail = {}, {}
+        for q in self.prjs:
+            s = "_%d" % q["id"]
+            ii = net.vector("ii" + s)
+            local[q["id"]] = ii.copy()
+            ej = net.vector("ej" + s)
+            hs = []
+            for h in range(q["H"]):
+                acc = np.float32(0)
+                for j in range(q["M"]):
+                    acc = np.float32(acc + ej[h * q["M"] + j])
+                free = bool(np.any(ii[h * q["C"]:(h + 1) * q["C"]] < 0))
+                if acc > hcu_thr and free:
+                    hs.append(h)
+            avail[q["id"]] = hs
+        draws = SpDraws(self.seed, step, lane)
+        for pop in self.pops:
+            cnt = net.vector(".counter_%d" % pop["id"])
+            active = [u for u in range(cnt.size) if cnt[u] > mcu_thr]
+            net.write(".counter_%d" % pop["id"], np.zeros_like(cnt))
+            fan = pop["fanout"]
+            prjs = list(pop["prjs"])
+            while prjs and active:
+                idx = int(draws.uniform() * np.float32(len(prjs)))
+                idx = min(idx, len(prjs) - 1)
+                q = self.prjs[prjs[idx]]
+                hcus = list(avail[q["id"]])
+                if not hcus:
+                    break
+                draws.shuffle(hcus)
+                buf = local[q["id"]]
+                C = q["C"]
+                for h in hcus:
+                    row = buf[h * C:(h + 1) * C]
+                    conn = set(int(c) for c in row if c >= 0)
+                    cand = [u for u in active if u not in conn]
+                    if not cand:
+                        continue
+                    draws.shuffle(cand)
+                    l = 0
+                    for k in range(C):
+                        if l >= len(cand):
+                            break
+                        if row[k] < 0:
+                            row[k] = cand[l]
+                            l += 1
+                    for u in cand[:l]:
+                        fan[u] -= 1
+                        if fan[u] <= 0:
+                            active = [a for a in active if a != u]
+                prjs.pop(idx)
+        delays = SpDraws(self.seed, step, delay_lane)
+        for q in self.prjs:
+            s = "_%d" % q["id"]
+            ii, di = net.vector("ii" + s), net.vector("di" + s)
+            new = (ii < 0) & (local[q["id"]] >= 0)
+            if not new.any():
+                continue
+            for index in np.nonzero(new)[0]:
+                h = int(index) // q["C"]
+                src = int(local[q["id"]][index])
+                # add_row (ProcStructuralPlasitcity.cpp:566-576): first free slot
+                k0 = int(np.nonzero(ii[h * q["C"]:(h + 1) * q["C"]] < 0)[0][0])
+                ii[h * q["C"] + k0] = src
+                di[h * q["C"] + k0] = self._delay(delays, q, src, h) if self.geometry else 1
+            net.write("ii" + s, ii)
+            net.write("di" + s, di)
+
+
 class OracleNet:
     """A network in the oracle, built from a gsbn_amd.configs.NetSpec."""
 
@@ -332,6 +586,7 @@ class OracleNet:
         self._mask = np.ascontiguousarray(spec.mask, dtype=np.float32)
         self._keep = []
         self._init_conn()
+        self.sp = StructuralPlasticity(spec, self) if getattr(spec, "structural", None) else None
         self.trace_shifts = []
         prec = TRACE_FORMS[getattr(spec, "trace_precision", "fp32")]
         if prec and arith != ARITH_F64:
@@ -406,8 +661,8 @@ class OracleNet:
             raise ValueError("write %s failed (%d, expected %d elements)" % (name, rc, n))
 
     def step(self) -> int:
-        """One Upd::update: ExtGen, ProcUpdLazy, ProcExchangeSpike, ProcCheck.
-        Returns cycle-flag."""
+        """One Upd::update: ExtGen, ProcUpdLazy, ProcExchangeSpike,
+        [ProcStructuralPlasticity], ProcCheck.  Returns cycle-flag."""
         L = lib()
         eg = self.extgen
         eg.update()
@@ -420,6 +675,8 @@ class OracleNet:
             spikes = [self.vector("spike_%d" % i) for i in range(len(self.spec.pops))]
         else:
             spikes = []
+        if self.sp is not None:
+            self.sp.update(eg.simstep, eg.cycle_flag, eg.plasticity)
         self.check.update(eg.cycle_flag, eg.simstep, spikes)
         return eg.cycle_flag
 
