@@ -17,7 +17,9 @@ roofline: the row/column kernel, timed with HIP events recorded by each of
 its launches on the solver stream over up to 1000 steps that follow the
 timed region (the timed steps themselves run uninstrumented); algorithmic
 bytes are 44 per synapse update (read pij, eij, zi2, zj2, tij; write them
-and wij).
+and wij).  When the online tuner chose the fused step launch (population
+and row/column update in one kernel), that kernel is the one timed and its
+algorithmic bytes add 40 per unit for the population update.
 
 forms: the same workload with Pij/Eij stored in bf16 and in scaled fp16
 (36 algorithmic bytes per update), same steps, reported beside the plain
@@ -49,6 +51,7 @@ METRIC = ("synapse-trace updates/sec + sim steps/sec at 100×100 HCU×MCU, plain
 # and writes them and wij (row) or reads/writes the five and reads ii
 # (column); 16-bit forms move 2-byte pij/eij
 BYTES_PER_SYN_UPDATE = {"fp32": 44, "bf16": 36, "fp16": 36}
+POP_BYTES_PER_UNIT = 40  # population update: dsup, act, epsc, bj, pj, ej, zj, counter, spike
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense bf16
 
@@ -132,7 +135,7 @@ def measure(spec, steps, warmup, step_times=0):
         # which run without instrumentation)
         s.timing(-1, collect=False)
         k0 = s.work()
-        s.timing(1 << 1, collect=False)
+        s.timing((1 << 1) | (1 << 6), collect=False)
         s.step(min(steps, 1000))
         s.sync()
         launches, ms = s.timing(0)
@@ -154,15 +157,20 @@ def measure(spec, steps, warmup, step_times=0):
     cols = int(w1[1] - w0[1])
     sweep = int(w1[2] - w0[2])
     bpu = BYTES_PER_SYN_UPDATE[spec.trace_precision]
-    rc_launch, rc_ms = int(launches[1]), float(ms[1])
+    fused = int(launches[6]) > int(launches[1])
+    kind = 6 if fused else 1
+    rc_launch, rc_ms = int(launches[kind]), float(ms[kind])
     avg_ms = rc_ms / max(rc_launch, 1)
     bytes_per_launch = bpu * int((k1[0] - k0[0]) + (k1[1] - k0[1])) / max(rc_launch, 1)
+    if fused:
+        bytes_per_launch += POP_BYTES_PER_UNIT * spec.total_units
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return {"elapsed": el, "done": int(done), "rows": rows, "cols": cols, "sweep": sweep,
             "updates_per_s": (rows + cols + sweep) / el, "steps_per_s": done / el,
             "ms_per_step": 1e3 * el / max(done, 1), "init_s": t_init, "step_ms": step_ms,
             "rc_launches": rc_launch, "rc_avg_ms": avg_ms, "rc_bytes_per_launch": bytes_per_launch,
-            "rc_achieved_gbs": achieved, "rc_parts": parts}
+            "rc_achieved_gbs": achieved, "rc_parts": abs(parts),
+            "rc_kernel": "step_kernel" if fused else "rowcol_kernel"}
 
 
 def copy_calibration(mib: int):
@@ -195,7 +203,7 @@ def batched(steps: int, warmup: int) -> dict:
             "synapse_updates_per_sec": m["updates_per_s"],
             "step_ms_p50": m["step_ms"]["p50"], "step_ms_p99": m["step_ms"]["p99"],
             "rowcol_avg_launch_ms": m["rc_avg_ms"], "rowcol_achieved_gbs": m["rc_achieved_gbs"],
-            "rowcol_workgroups_per_hcu": m["rc_parts"]}
+            "rowcol_workgroups_per_hcu": m["rc_parts"], "timed_kernel": m["rc_kernel"]}
 
 
 def frozen_recall(batch: int = 8192, steps: int = 50, clamp_frac: float = 0.4) -> dict:
@@ -314,7 +322,7 @@ def main():
         "init_s": m["init_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "rowcol_kernel", "avg_launch_ms": m["rc_avg_ms"],
+                     "kernel": m["rc_kernel"], "avg_launch_ms": m["rc_avg_ms"],
                      "launches": m["rc_launches"], "bytes_per_launch": m["rc_bytes_per_launch"],
                      "workgroups_per_hcu": m["rc_parts"]},
     }
@@ -333,6 +341,7 @@ def main():
                            "speedup_steps_per_sec": f["steps_per_s"] / m["steps_per_s"],
                            "rowcol_avg_launch_ms": f["rc_avg_ms"],
                            "rowcol_achieved_gbs": f["rc_achieved_gbs"],
+                           "timed_kernel": f["rc_kernel"],
                            "bytes_per_update": BYTES_PER_SYN_UPDATE[form]}
         line["forms"] = forms
     if args.workload == "100x100" and not args.no_batched:

@@ -175,8 +175,10 @@ class Solver:
             raise _err("snapshot")
 
     def timing(self, enable: int, collect: bool = True):
-        n = np.zeros(6, np.int64)
-        ms = np.zeros(6, np.float64)
+        """Per-kind launch counts and total ms (kinds: 0 pop, 1 rowcol, 2 all,
+        3 jxx, 4 que, 5 check, 6 fused step)."""
+        n = np.zeros(8, np.int64)
+        ms = np.zeros(8, np.float64)
         if lib().gsbn_solver_timing(self._h, enable, _p(n) if collect else None,
                                     _p(ms) if collect else None) != 0:
             raise _err("timing")

@@ -126,6 +126,11 @@ class NetSpec:
     snapshot: Optional[Tuple[int, int]] = None
     spike_record: Optional[Tuple[int, int]] = None
     structural: Optional[StructuralSpec] = None  # ProcStructuralPlasticity after the exchange
+    # ProcUpdLazy step plan: fused_step 1 runs the population and row/column
+    # updates as one launch per step; rowcol_parts > 0 fixes the split
+    # launches' work split; both 0: chosen online
+    fused_step: int = 0
+    rowcol_parts: int = 0
 
     @property
     def total_hcu(self) -> int:
@@ -280,6 +285,10 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('  args { key: "external spike" val: "%s" }' % ext_spike_path)
     if spec.trace_precision != "fp32":
         L.append('  args { key: "trace-precision" val: "%s" }' % spec.trace_precision)
+    if spec.fused_step:
+        L.append('  argi { key: "fused-step" val: 1 }')
+    if spec.rowcol_parts:
+        L.append('  argi { key: "rowcol-parts" val: %d }' % spec.rowcol_parts)
     L.append("}")
     if spec.conn_arrays is None:
         L.append("proc_param {")

@@ -275,11 +275,11 @@ int gsbn_solver_work(gsbn_solver* s, int64_t* out5) {
   });
 }
 
-int gsbn_solver_timing(gsbn_solver* s, int enable, int64_t* launches6, double* ms6) {
+int gsbn_solver_timing(gsbn_solver* s, int enable, int64_t* launches8, double* ms8) {
   return guard([&] {
     if (!s || !s->s) throw Error(GSBN_ERR_ARG, "null solver");
     auto& t = gsbn::Timing::get();
-    if (launches6 && ms6) t.collect(launches6, ms6, 6);
+    if (launches8 && ms8) t.collect(launches8, ms8, 8);
     if (enable < 0) t.reset();
     t.enable(enable > 0 ? enable : 0);
     return 0;

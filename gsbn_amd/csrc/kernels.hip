@@ -190,7 +190,25 @@ __global__ void pop_dsup0_kernel(gsbn_pop_args a) {
   a.scratch_dsup0[0] = unit_dsup(a, 0, 0, efin, &u, &xi);
 }
 
-__device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
+/* What the fused step's row/column phase takes from its population phase
+ * (all in LDS): the spike flags, the post-jxx pj and the spike list of the
+ * hypercolumn, and the active-row bitmap and list of its one incoming
+ * projection. */
+struct FusedLds {
+  const int* sj;       /* [M] */
+  const float* pj;     /* [M] */
+  const int* spk;      /* [nspk] */
+  int nspk;
+  const uint32_t* bitmap;
+  const int* rows;     /* [nrows], ascending */
+  int nrows;
+};
+
+/* One hypercolumn's population update.  FUSED: `f` also receives what the
+ * fused step's row/column phase needs (one incoming projection). */
+template <bool FUSED>
+__device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
+                                         FusedLds* f = nullptr) {
   extern __shared__ uint32_t sh_bitmap[];
   __shared__ float sh_f[POP_MAX_THREADS];
   __shared__ int sh_ext[POP_MAX_THREADS];
@@ -198,6 +216,8 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
   __shared__ int sh_scan[POP_MAX_THREADS / 64 + 1];
   __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
   __shared__ float sh_vsum;
+  __shared__ int sh_sj[FUSED ? RC_THREADS : 1];
+  __shared__ float sh_pj[FUSED ? RC_THREADS : 1];
 
   const int j = threadIdx.x;
   const int M = a.M;
@@ -374,6 +394,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
     if (live) spk = sh_ext[j];
   }
   if (live) a.spike[idx] = (int8_t)spk;
+  if (FUSED && live) sh_sj[j] = spk;
 
   /* 5. spiking units of this hypercolumn, ascending (.ssj, Proj.cpp:632-643) */
   int spk_tot;
@@ -400,6 +421,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
 
   /* 6. postsynaptic traces and bias for every incoming projection
    * (update_jxx_kernel_cpu, Proj.cpp:310-363) */
+  if (FUSED && live) sh_pj[j] = a.do_jxx ? pj_n : pj0;
   if (a.do_jxx && live) {
     for (int m = 0; m < nin; ++m) {
       const gsbn_pop_proj& p = a.projs[m];
@@ -449,6 +471,8 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
     for (int w = w0; w < w1; ++w) c += __popc(sh_bitmap[w]);
     int tot;
     int pos = block_scan_fast(c, sh_scan, &tot);
+    /* FUSED: the list also goes to LDS after the bitmap */
+    int* rows = reinterpret_cast<int*>(sh_bitmap + W);
     for (int w = w0; w < w1; ++w) {
       uint32_t bits = sh_bitmap[w];
       while (bits) {
@@ -457,8 +481,18 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
         const int k = (w << 5) + b;
         p.rows_list[(int64_t)h * C + pos] = k;
         p.act_stamp[(int64_t)h * C + k] = a.exec;
+        if (FUSED) rows[pos] = k;
         ++pos;
       }
+    }
+    if (FUSED) {
+      f->sj = sh_sj;
+      f->pj = sh_pj;
+      f->spk = sh_spk;
+      f->nspk = spk_tot;
+      f->bitmap = sh_bitmap;
+      f->rows = rows;
+      f->nrows = tot;
     }
     if (j == 0) {
       p.rows_cnt[h] = tot;
@@ -475,7 +509,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h) {
 }
 
 __global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
-  pop_body(a, blockIdx.x);
+  pop_body<false>(a, blockIdx.x);
 }
 
 /* per-step fields shared by every population / projection of a batch */
@@ -493,7 +527,7 @@ __global__ void __launch_bounds__(POP_MAX_THREADS)
                      const int32_t* __restrict__ wg_hcu, gsbn_step_args st) {
   gsbn_pop_args a = arr[wg_pop[blockIdx.x]];
   apply_step(a, st);
-  pop_body(a, wg_hcu[blockIdx.x]);
+  pop_body<false>(a, wg_hcu[blockIdx.x]);
 }
 
 __global__ void pop_dsup0_kernel_batch(const gsbn_pop_args* __restrict__ arr, int npop,
@@ -634,9 +668,13 @@ struct RcStep {
   const float* table;
 };
 
-template <int PREC>
+/* Part `part` of the row/column update of hypercolumn h.  FUSED: the
+ * hypercolumn's active rows, spikes and pj come from the population phase
+ * of the same workgroup (fl), not from memory. */
+template <int PREC, bool FUSED = false>
 __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcStep& q,
-                                            const int h, const int part) {
+                                            const int h, const int part,
+                                            const FusedLds* fl = nullptr) {
   const SynIO<PREC> io(a);
   const int P = q.P;
   const int M = a.M, C = a.C;
@@ -645,7 +683,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   const bool learn = q.kp != 0.0f;
 
   /* rows part, part+P, ... of this HCU's ascending active-row list */
-  const int nrows = a.rows_cnt[h];
+  const int nrows = FUSED ? fl->nrows : a.rows_cnt[h];
   const int mine = nrows > part ? (nrows - part + P - 1) / P : 0;
   const int rpt = M >= RC_THREADS ? 1 : RC_THREADS / M;
   for (int base = 0; base < mine; base += rpt) {
@@ -660,7 +698,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
       const int rr = f / M;
       const int j = f - rr * M;
       const int aa = part + P * (base + rr);
-      const int k = a.rows_list[(int64_t)h * C + aa];
+      const int k = FUSED ? fl->rows[aa] : a.rows_list[(int64_t)h * C + aa];
       const int64_t r = (int64_t)h * C + k;
       const int64_t sidx = r * M + j;
       gsbn_row_rec st = a.rowst[r];
@@ -678,35 +716,36 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
         st0 = st;
         r0 = r;
       }
-      float contrib;
+      float cval;
       bool w_new = false;
       const int D = t - s.tij;
       if (D <= 0) {
         s.zi2 += a.kfti;
         s.tij = t;
-        contrib = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
+        cval = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
       } else {
         gsbn_syn_catchup_k(load_coef(q.table, q.table_len, D), &s.pij, &s.eij, &s.zi2, &s.zj2);
         s.zi2 = s.zi2 + a.kfti;
         s.tij = t;
         if (learn) {
-          s.wij = gsbn_wij(s.pij, st.pi, a.pj[h * M + j], a.wgain, a.eps, a.eps2);
+          s.wij = gsbn_wij(s.pij, st.pi, FUSED ? fl->pj[j] : a.pj[h * M + j], a.wgain, a.eps,
+                           a.eps2);
           w_new = true;
         }
-        contrib = s.wij;
+        cval = s.wij;
       }
       /* column increment of a spiking postsynaptic unit after the row
        * update (the reference's col pass sees D == 0 here) */
-      if (a.sj[h * M + j] > 0) s.zj2 += a.kftj;
+      if ((FUSED ? fl->sj[j] : a.sj[h * M + j]) > 0) s.zj2 += a.kftj;
       io.store(sidx, s, w_new);
-      a.contrib[((int64_t)h * C + aa) * M + j] = contrib;
+      a.contrib[((int64_t)h * C + aa) * M + j] = cval;
     }
     __syncthreads();
     if (r0 >= 0) a.rowst[r0] = st0;
   }
 
   /* slice `part` of the slots of every spiking unit of this HCU */
-  const int nspk = a.spk_cnt[h];
+  const int nspk = FUSED ? fl->nspk : a.spk_cnt[h];
   if (nspk == 0) return;
   const int slice = (C + P - 1) / P;
   const int k0 = part * slice;
@@ -716,7 +755,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
    * coefficients, then the updates */
   constexpr int U = RC_COLS;
   for (int s_i = 0; s_i < nspk; ++s_i) {
-    const int j = a.spk_list[h * M + s_i];
+    const int j = FUSED ? fl->spk[s_i] : a.spk_list[h * M + s_i];
     for (int kb = k0 + tid; kb < k1; kb += U * RC_THREADS) {
       bool ok[U];
       int64_t sidx[U];
@@ -728,7 +767,13 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
         const int r = h * C + (k < k1 ? k : k0);
         sidx[u] = (int64_t)r * M + j;
         iiv[u] = a.ii[r];
-        stamp[u] = a.act_stamp[r];
+        /* rows updated by this step's row pass are skipped */
+        if (FUSED) {
+          const int kk = r - h * C;
+          stamp[u] = ((fl->bitmap[kk >> 5] >> (kk & 31)) & 1u) ? q.exec : q.exec - 1;
+        } else {
+          stamp[u] = a.act_stamp[r];
+        }
         ok[u] = k < k1;
         /* the record is loaded with the slot checks (a skipped slot only
          * costs an unused load) */
@@ -778,6 +823,38 @@ __global__ void __launch_bounds__(RC_THREADS)
   const RcStep q = {st.simstep, st.exec, parts, a.table_lens[st.kp_slot], a.taupdt * st.prn,
                     a.tables[st.kp_slot]};
   rowcol_body<PREC>(a, q, wg_hcu[w], blockIdx.x - w * parts);
+}
+
+/* One launch per step: workgroup h updates hypercolumn h's population and
+ * then, from LDS, its rows and columns.  Everything the row/column update
+ * reads in a step comes from the same workgroup and the queue pushes go to
+ * slots at least three steps ahead, so no workgroup waits for another. */
+template <int PREC>
+__global__ void __launch_bounds__(RC_THREADS) step_kernel(gsbn_pop_args a, gsbn_rowcol_args b) {
+  const int h = blockIdx.x;
+  FusedLds f;
+  pop_body<true>(a, h, &f);
+  __syncthreads();
+  const RcStep q = {b.simstep, b.exec, 1, b.table_len, b.kp, b.table};
+  rowcol_body<PREC, true>(b, q, h, 0, &f);
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(RC_THREADS)
+    step_kernel_batch(const gsbn_pop_args* __restrict__ pops, const gsbn_rowcol_args* __restrict__ projs,
+                      const int32_t* __restrict__ wg_pop, const int32_t* __restrict__ wg_hcu,
+                      const int32_t* __restrict__ in_proj, gsbn_step_args st) {
+  const int p = wg_pop[blockIdx.x];
+  const int h = wg_hcu[blockIdx.x];
+  gsbn_pop_args a = pops[p];
+  apply_step(a, st);
+  FusedLds f;
+  pop_body<true>(a, h, &f);
+  __syncthreads();
+  const gsbn_rowcol_args& b = projs[in_proj[p]];
+  const RcStep q = {st.simstep, st.exec, 1, b.table_lens[st.kp_slot], b.taupdt * st.prn,
+                    b.tables[st.kp_slot]};
+  rowcol_body<PREC, true>(b, q, h, 0, &f);
 }
 
 template <int PREC>
@@ -1236,6 +1313,78 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
     default:
       launch_timed(rowcol_kernel_batch<GSBN_PREC_F16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
                          parts, *step);
+      break;
+  }
+  return launch_status();
+}
+
+/* dynamic LDS of the fused step: the active-row bitmap and list */
+static size_t fused_lds(int64_t max_slots) {
+  return (size_t)((max_slots + 31) / 32 + max_slots) * sizeof(uint32_t);
+}
+
+int gsbn_step_fused(const gsbn_pop_args* a, const gsbn_rowcol_args* b, void* stream) {
+  if (!a || !b || a->H <= 0 || a->M <= 0 || a->M > RC_THREADS || a->nproj_in != 1 || !a->projs ||
+      a->max_slots < b->C || a->nproj_out < 0 || (a->nproj_out > 0 && !a->outs) || b->H != a->H ||
+      b->M != a->M || b->C < 0 || !b->table || b->table_len < 2 || !b->contrib ||
+      !trace_form_ok(*b))
+    return GSBN_ERR_ARG;
+  if (a->wta_pop_max && !a->scratch_dsup0) return GSBN_ERR_ARG;
+  const size_t lds = fused_lds(a->max_slots);
+  if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
+  hipStream_t s = (hipStream_t)stream;
+  if (a->wta_pop_max) {
+    hipLaunchKernelGGL(pop_dsup0_kernel, dim3(1), dim3(64), 0, s, *a);
+    int rc = launch_status();
+    if (rc) return rc;
+  }
+  const dim3 g((unsigned)a->H), t(RC_THREADS);
+  switch (b->prec) {
+    case GSBN_PREC_F32:
+      launch_timed(step_kernel<GSBN_PREC_F32>, g, t, lds, s, *a, *b);
+      break;
+    case GSBN_PREC_BF16:
+      launch_timed(step_kernel<GSBN_PREC_BF16>, g, t, lds, s, *a, *b);
+      break;
+    default:
+      launch_timed(step_kernel<GSBN_PREC_F16>, g, t, lds, s, *a, *b);
+      break;
+  }
+  return launch_status();
+}
+
+int gsbn_step_fused_batch(const gsbn_pop_args* pops_dev, int32_t npop,
+                          const gsbn_rowcol_args* projs_dev, const int32_t* wg_pop,
+                          const int32_t* wg_hcu, int32_t nwg, const int32_t* in_proj,
+                          int32_t max_m, int32_t max_slots, int32_t any_wta_pop_max,
+                          const gsbn_step_args* step, void* stream) {
+  if (!pops_dev || !projs_dev || !wg_pop || !wg_hcu || !in_proj || !step || nwg <= 0 ||
+      npop <= 0 || max_m <= 0 || max_m > RC_THREADS || max_slots < 0 ||
+      step->kp_slot < 0 || step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 ||
+      step->prec > GSBN_PREC_F16)
+    return GSBN_ERR_ARG;
+  const size_t lds = fused_lds(max_slots);
+  if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
+  hipStream_t s = (hipStream_t)stream;
+  if (any_wta_pop_max) {
+    hipLaunchKernelGGL(pop_dsup0_kernel_batch, dim3((npop + 63) / 64), dim3(64), 0, s, pops_dev,
+                       npop, *step);
+    int rc = launch_status();
+    if (rc) return rc;
+  }
+  const dim3 g((unsigned)nwg), b(RC_THREADS);
+  switch (step->prec) {
+    case GSBN_PREC_F32:
+      launch_timed(step_kernel_batch<GSBN_PREC_F32>, g, b, lds, s, pops_dev, projs_dev, wg_pop,
+                   wg_hcu, in_proj, *step);
+      break;
+    case GSBN_PREC_BF16:
+      launch_timed(step_kernel_batch<GSBN_PREC_BF16>, g, b, lds, s, pops_dev, projs_dev, wg_pop,
+                   wg_hcu, in_proj, *step);
+      break;
+    default:
+      launch_timed(step_kernel_batch<GSBN_PREC_F16>, g, b, lds, s, pops_dev, projs_dev, wg_pop,
+                   wg_hcu, in_proj, *step);
       break;
   }
   return launch_status();

@@ -567,7 +567,6 @@ gsbn_pop_args Pop::args(int simstep, int exec, float prn, int lginp_idx, int wma
 void Pop::flush() {
   if (!flush_pending) return;
   flush_pending = false;
-  int simstep = 0;
   gsbn_pop_args a;
   memset(&a, 0, sizeof(a));
   a.H = _dim_hcu;
@@ -575,7 +574,6 @@ void Pop::flush() {
   a.epsc = _epsc->mutable_gpu_data();
   a.nproj_in = (int)_in.size();
   a.projs = projs_dev.p;
-  (void)simstep;
   if (a.nproj_in && a.projs)
     op_check(gsbn_pop_flush(&a, active_stream()), "gsbn_pop_flush");
 }
@@ -1116,14 +1114,21 @@ class ProcUpdLazy : public ProcedureBase {
   /* batched launch state */
   DevBuf<gsbn_pop_args> _pop_dev;
   DevBuf<gsbn_rowcol_args> _proj_dev;
-  DevBuf<int32_t> _wg_pop, _wg_pop_hcu, _wg_proj, _wg_proj_hcu;
+  DevBuf<int32_t> _wg_pop, _wg_pop_hcu, _wg_proj, _wg_proj_hcu, _in_proj;
   int _pop_threads = 64, _max_slots = 0, _any_quirk = 0, _nwg_pop = 0, _nhcu_proj = 0, _parts = 1;
-  /* row/column work split: workgroups per destination HCU.  Fixed by argi
-   * rowcol-parts, or chosen online: from step kTuneStart on, each candidate
-   * is used for kTuneSamples executed steps in turn (results do not depend
-   * on the split) and the one with the smallest median kernel time is kept */
+  int _max_m = 0, _parts0 = 1;
+  /* every population has M <= 256 and one incoming projection, and the
+   * fused step's LDS (active-row bitmap and list) fits */
+  bool _fusable = false;
+  /* Step plan: the row/column work split (workgroups per destination HCU),
+   * or -1 for the fused launch (population and row/column update of an HCU
+   * in one workgroup, one launch per step).  Fixed by argi rowcol-parts, or
+   * chosen online: from step kTuneStart on, each candidate is used for
+   * kTuneSamples executed steps without a sweep in turn (results do not
+   * depend on the plan) and the one with the smallest median step time on
+   * the device is kept */
   static constexpr int kTuneStart = 20, kTuneSamples = 6;
-  int _parts_fixed = 0;
+  int _parts_fixed = 0, _fused_fixed = 0;
   bool _tuned = false;
   int _tune_pos = 0;
   vector<int> _cand;
@@ -1143,12 +1148,14 @@ ProcUpdLazy::~ProcUpdLazy() {
  * tuning sample (events _tev[2i], _tev[2i+1] are to be recorded around it) */
 int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
   *timed = false;
+  if (_fused_fixed) return -1;
   if (_parts_fixed > 0) return _parts_fixed;
   if (_tuned || execs < kTuneStart) return _parts;
   if (_cand.empty()) {
     for (int c : {1, 2, 4, 8, 16, 32})
       if ((int64_t)c * std::max(_nhcu_proj, 1) <= 262144) _cand.push_back(c);
     if (_cand.empty()) _cand.push_back(1);
+    if (_fusable) _cand.push_back(-1);
     _tev.resize(2 * _cand.size() * kTuneSamples);
     for (auto& e : _tev) hip_check(hipEventCreate(&e), "hipEventCreate");
   }
@@ -1171,6 +1178,8 @@ void ProcUpdLazy::finish_tuning() {
     }
     std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
     const float med = t[t.size() / 2];
+    /* the measured median step time of each plan, for inspection */
+    _glv.putf("step-plan-ms" + std::to_string(_cand[c]), med);
     if (c == 0 || med < best) {
       best = med;
       _parts = _cand[c];
@@ -1191,6 +1200,7 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   _seed = (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 31);
   par.argi("wta-pop-max", _wta_pop_max);
   par.argi("rowcol-parts", _parts_fixed);
+  par.argi("fused-step", _fused_fixed);
   if (_parts_fixed < 0 || _parts_fixed > 1024)
     throw Error(-2, "ProcUpdLazy: rowcol-parts must be in [0, 1024]");
   int hcu_cnt = 0, mcu_cnt = 0;
@@ -1266,6 +1276,13 @@ void ProcUpdLazy::build_batch_maps() {
   }
   _nwg_pop = (int)wp.size();
   _nhcu_proj = (int)qp.size();
+  /* the projection into each population (fused step: exactly one) */
+  vector<int32_t> in;
+  for (Pop* p : _list_pop)
+    in.push_back(p->_in.empty() ? 0
+                                : (int32_t)(std::find(_list_proj.begin(), _list_proj.end(),
+                                                      p->_in[0]) - _list_proj.begin()));
+  _in_proj.upload(in.data(), in.size());
   _wg_pop.upload(wp.data(), wp.size());
   _wg_pop_hcu.upload(wh.data(), wh.size());
   if (!qp.empty()) {
@@ -1313,9 +1330,24 @@ void ProcUpdLazy::update_gpu() {
     if (_batch) build_batch_maps();
     _nhcu_proj = 0;
     for (auto* q : _list_proj) _nhcu_proj += q->_dim_hcu;
+    _max_m = 0;
+    _fusable = !_list_proj.empty();
+    for (auto* p : _list_pop) {
+      _max_m = std::max(_max_m, p->_dim_mcu);
+      if (p->_dim_mcu > 256 || p->_in.size() != 1) _fusable = false;
+    }
+    if (!_batch) {
+      _max_slots = 0;
+      for (auto* q : _list_proj) _max_slots = std::max(_max_slots, q->_dim_conn);
+    }
+    if (((int64_t)(_max_slots + 31) / 32 + _max_slots) * 4 > 48 * 1024) _fusable = false;
+    if (_fused_fixed && !_fusable)
+      throw Error(-2, "ProcUpdLazy: fused-step needs populations of at most 256 MCUs with one "
+                      "incoming projection each, and the slot bitmap and list in 48 KB of LDS");
     /* starting split until the tuning has run: about 1536 workgroups */
-    _parts = std::max(1, std::min(32, 1536 / std::max(_nhcu_proj, 1)));
-    _glv.puti("rowcol-parts", _parts_fixed > 0 ? _parts_fixed : _parts);
+    _parts0 = std::max(1, std::min(32, 1536 / std::max(_nhcu_proj, 1)));
+    _parts = _fused_fixed ? -1 : _parts0;
+    _glv.puti("rowcol-parts", _fused_fixed ? -1 : _parts_fixed > 0 ? _parts_fixed : _parts);
     _configured = true;
     _args_dirty = true;
   }
@@ -1349,56 +1381,81 @@ void ProcUpdLazy::update_gpu() {
   st.lginp_offset = lg_off;
   st.wmask_offset = wm_off;
 
-  Timing::get().begin(0, stream);
-  if (_batch) {
-    op_check(gsbn_pop_update_batch(_pop_dev.p, (int32_t)_pop_args.size(), _wg_pop.p, _wg_pop_hcu.p,
-                                   _nwg_pop, _pop_threads, _max_slots, _any_quirk, &st, stream),
-             "gsbn_pop_update_batch");
-  } else {
-    gsbn_pop_args a = _pop_args[0];
-    a.simstep = simstep;
-    a.exec = _exec;
-    a.prn = prn;
-    a.do_jxx = st.do_jxx;
-    a.lginp += lg_off;
-    a.wmask += wm_off;
-    op_check(gsbn_pop_update(&a, stream), "gsbn_pop_update");
-  }
-  Timing::get().end(0, stream);
-  for (auto* p : _list_pop) p->flush_pending = true;
+  /* the step plan (row/column parts, or -1: the fused launch); sweep steps
+   * run the sweep between the two launches and are not tuning samples */
+  bool timed = false;
+  int plan = _list_proj.empty() ? 1 : rowcol_parts(stream, &timed);
   if (sweep) {
-    for (auto* q : _list_proj) {
-      q->update_all(simstep, old_prn);
-      sweep_synapses += (int64_t)q->_dim_hcu * q->_dim_conn * q->_dim_mcu;
-    }
-    sweeps++;
-    for (auto* q : _list_proj) q->update_jxx(prn);
+    timed = false;
+    if (plan < 0) plan = _parts0;
   }
-  if (!_list_proj.empty()) {
-    bool timed = false;
-    const int parts = rowcol_parts(stream, &timed);
-    const size_t ev = 2 * (size_t)_tune_pos;
-    if (timed) hip_check(hipEventRecord(_tev[ev], stream), "hipEventRecord");
-    Timing::get().begin(1, stream);
-    if (_batch) {
-      op_check(gsbn_proj_rowcol_batch(_proj_dev.p, _wg_proj.p, _wg_proj_hcu.p, _nhcu_proj, parts,
-                                      &st, stream),
-               "gsbn_proj_rowcol_batch");
-    } else {
-      gsbn_rowcol_args a = _proj_args[0];
-      a.simstep = simstep;
-      a.exec = _exec;
-      a.kp = a.taupdt * prn;
-      a.table = a.tables[slot];
-      a.table_len = a.table_lens[slot];
-      a.parts = parts;
-      op_check(gsbn_proj_rowcol(&a, stream), "gsbn_proj_rowcol");
+  const size_t ev = 2 * (size_t)_tune_pos;
+  if (timed) hip_check(hipEventRecord(_tev[ev], stream), "hipEventRecord");
+
+  gsbn_pop_args pa;
+  if (!_batch) {
+    pa = _pop_args[0];
+    pa.simstep = simstep;
+    pa.exec = _exec;
+    pa.prn = prn;
+    pa.do_jxx = st.do_jxx;
+    pa.lginp += lg_off;
+    pa.wmask += wm_off;
+  }
+  gsbn_rowcol_args ra;
+  if (!_batch && !_list_proj.empty()) {
+    ra = _proj_args[0];
+    ra.simstep = simstep;
+    ra.exec = _exec;
+    ra.kp = ra.taupdt * prn;
+    ra.table = ra.tables[slot];
+    ra.table_len = ra.table_lens[slot];
+    ra.parts = plan < 0 ? 1 : plan;
+  }
+  if (plan < 0) {
+    Timing::get().begin(6, stream);
+    if (_batch)
+      op_check(gsbn_step_fused_batch(_pop_dev.p, (int32_t)_pop_args.size(), _proj_dev.p, _wg_pop.p,
+                                     _wg_pop_hcu.p, _nwg_pop, _in_proj.p, _max_m, _max_slots,
+                                     _any_quirk, &st, stream),
+               "gsbn_step_fused_batch");
+    else
+      op_check(gsbn_step_fused(&pa, &ra, stream), "gsbn_step_fused");
+    Timing::get().end(6, stream);
+    for (auto* p : _list_pop) p->flush_pending = true;
+  } else {
+    Timing::get().begin(0, stream);
+    if (_batch)
+      op_check(gsbn_pop_update_batch(_pop_dev.p, (int32_t)_pop_args.size(), _wg_pop.p,
+                                     _wg_pop_hcu.p, _nwg_pop, _pop_threads, _max_slots, _any_quirk,
+                                     &st, stream),
+               "gsbn_pop_update_batch");
+    else
+      op_check(gsbn_pop_update(&pa, stream), "gsbn_pop_update");
+    Timing::get().end(0, stream);
+    for (auto* p : _list_pop) p->flush_pending = true;
+    if (sweep) {
+      for (auto* q : _list_proj) {
+        q->update_all(simstep, old_prn);
+        sweep_synapses += (int64_t)q->_dim_hcu * q->_dim_conn * q->_dim_mcu;
+      }
+      sweeps++;
+      for (auto* q : _list_proj) q->update_jxx(prn);
     }
-    Timing::get().end(1, stream);
-    if (timed) {
-      hip_check(hipEventRecord(_tev[ev + 1], stream), "hipEventRecord");
-      if (++_tune_pos == (int)(_cand.size() * kTuneSamples)) finish_tuning();
+    if (!_list_proj.empty()) {
+      Timing::get().begin(1, stream);
+      if (_batch)
+        op_check(gsbn_proj_rowcol_batch(_proj_dev.p, _wg_proj.p, _wg_proj_hcu.p, _nhcu_proj, plan,
+                                        &st, stream),
+                 "gsbn_proj_rowcol_batch");
+      else
+        op_check(gsbn_proj_rowcol(&ra, stream), "gsbn_proj_rowcol");
+      Timing::get().end(1, stream);
     }
+  }
+  if (timed) {
+    hip_check(hipEventRecord(_tev[ev + 1], stream), "hipEventRecord");
+    if (++_tune_pos == (int)(_cand.size() * kTuneSamples)) finish_tuning();
   }
   for (auto* q : _list_proj) q->update_que(_exec);
   if (touched)

@@ -242,6 +242,25 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
                            const int32_t* wg_hcu, int32_t nhcu, int32_t parts,
                            const gsbn_step_args* step, void* stream);
 
+/* One launch per executed step without a sweep (ProcUpdLazy.cpp:112-123
+ * and 153-165 in one kernel), for populations with one incoming projection
+ * and M <= 256: workgroup h runs the population update of hypercolumn h and
+ * then the whole row/column update of h (gsbn_proj_rowcol with one part),
+ * taking the hypercolumn's spikes, pj and active rows from its own LDS.
+ * Results equal the two separate launches.  (Several workgroups per
+ * hypercolumn would each need the population's inputs while one of them
+ * overwrites them in place, so the fused form has one.)  Single form: one
+ * population and its projection.  Batched form: population wg_pop[w],
+ * hypercolumn wg_hcu[w] for w < nwg, with the rowcol arguments
+ * projs_dev[in_proj[p]] of the projection into population p; max_m is the
+ * largest M, max_slots the largest slot count. */
+int gsbn_step_fused(const gsbn_pop_args* pop, const gsbn_rowcol_args* proj, void* stream);
+int gsbn_step_fused_batch(const gsbn_pop_args* pops_dev, int32_t npop,
+                          const gsbn_rowcol_args* projs_dev, const int32_t* wg_pop,
+                          const int32_t* wg_hcu, int32_t nwg, const int32_t* in_proj,
+                          int32_t max_m, int32_t max_slots, int32_t any_wta_pop_max,
+                          const gsbn_step_args* step, void* stream);
+
 /* Delay-queue push (Proj.cpp:645-664) from si: si holds the spikes of exec
  * step exec - 1 (ProcExchangeSpike's copy), each source unit with si > 0
  * pushes its rows.  Grid: one workgroup per source HCU. */
@@ -423,9 +442,10 @@ int gsbn_solver_check(gsbn_solver* s, int32_t* correct, int32_t* total, int32_t*
 int gsbn_solver_work(gsbn_solver* s, int64_t* out5);
 /* per-kernel timing with HIP events on the solver stream.  enable > 0 is a
  * bit mask of the kinds to time (0 pop, 1 rowcol, 2 all, 3 jxx, 4 que, 5
- * check), 0 stops timing, < 0 also clears the totals; when launches6/ms6 are
- * given the totals {launches, total_ms} per kind are written first. */
-int gsbn_solver_timing(gsbn_solver* s, int enable, int64_t* launches6, double* ms6);
+ * check, 6 fused step, 7 unused), 0 stops timing, < 0 also clears the
+ * totals; when launches8/ms8 are given the totals {launches, total_ms} per
+ * kind are written first. */
+int gsbn_solver_timing(gsbn_solver* s, int enable, int64_t* launches8, double* ms8);
 
 /* write the solver's state now, as ProcSnapshot does at its steps
  * (ProcSnapshot.cpp:69-78): timestamp simstep*dt, the current prn */
