@@ -1123,14 +1123,17 @@ class ProcUpdLazy : public ProcedureBase {
   /* Step plan: the row/column work split (workgroups per destination HCU),
    * or -1 for the fused launch (population and row/column update of an HCU
    * in one workgroup, one launch per step).  Fixed by argi rowcol-parts, or
-   * chosen online: from step kTuneStart on, each candidate is used for
-   * kTuneSamples executed steps without a sweep in turn (results do not
-   * depend on the plan) and the one with the smallest median step time on
-   * the device is kept */
-  static constexpr int kTuneStart = 20, kTuneSamples = 6;
+   * chosen online: from step kTuneStart on, the candidates take turns step
+   * by step (so that each sees the same mix of busy and quiet steps; results
+   * do not depend on the plan), a step without a sweep timed on the device
+   * from its first launch to its last; after kTuneSamples steps each, the
+   * plan with the smallest median step time is kept.  (Runs of several
+   * consecutive steps per sample were tried: they met different spiking
+   * activity per candidate and chose worse plans.) */
+  static constexpr int kTuneStart = 20, kTuneRun = 1, kTuneSamples = 10;
   int _parts_fixed = 0, _fused_fixed = 0;
   bool _tuned = false;
-  int _tune_pos = 0;
+  int _tune_pos = 0, _run_len = 0;
   vector<int> _cand;
   vector<hipEvent_t> _tev;
   int rowcol_parts(hipStream_t s, bool* timed);
@@ -1144,8 +1147,8 @@ ProcUpdLazy::~ProcUpdLazy() {
   for (auto* p : _list_pop) delete p;
 }
 
-/* parts for this step's row/column launch; *timed when the launch is a
- * tuning sample (events _tev[2i], _tev[2i+1] are to be recorded around it) */
+/* the plan of this step; *timed while tuning (the step belongs to run
+ * _tune_pos, timed by events _tev[2 _tune_pos] and _tev[2 _tune_pos + 1]) */
 int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
   *timed = false;
   if (_fused_fixed) return -1;
@@ -1174,7 +1177,7 @@ void ProcUpdLazy::finish_tuning() {
       const size_t i = (size_t)k * nc + c;
       float ms = 0.0f;
       hip_check(hipEventElapsedTime(&ms, _tev[2 * i], _tev[2 * i + 1]), "hipEventElapsedTime");
-      t.push_back(ms);
+      t.push_back(ms / kTuneRun);
     }
     std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
     const float med = t[t.size() / 2];
@@ -1386,11 +1389,12 @@ void ProcUpdLazy::update_gpu() {
   bool timed = false;
   int plan = _list_proj.empty() ? 1 : rowcol_parts(stream, &timed);
   if (sweep) {
+    if (timed) _run_len = 0; /* the run restarts after the sweep */
     timed = false;
     if (plan < 0) plan = _parts0;
   }
   const size_t ev = 2 * (size_t)_tune_pos;
-  if (timed) hip_check(hipEventRecord(_tev[ev], stream), "hipEventRecord");
+  if (timed && _run_len == 0) hip_check(hipEventRecord(_tev[ev], stream), "hipEventRecord");
 
   gsbn_pop_args pa;
   if (!_batch) {
@@ -1453,11 +1457,12 @@ void ProcUpdLazy::update_gpu() {
       Timing::get().end(1, stream);
     }
   }
-  if (timed) {
+  for (auto* q : _list_proj) q->update_que(_exec);
+  if (timed && ++_run_len == kTuneRun) {
     hip_check(hipEventRecord(_tev[ev + 1], stream), "hipEventRecord");
+    _run_len = 0;
     if (++_tune_pos == (int)(_cand.size() * kTuneSamples)) finish_tuning();
   }
-  for (auto* q : _list_proj) q->update_que(_exec);
   if (touched)
     for (auto* q : _list_proj) q->mark_views_device();
   _epoch_seen = VecBase::host_epoch();
