@@ -209,10 +209,13 @@ struct FusedLds {
 template <bool FUSED>
 __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
                                          FusedLds* f = nullptr) {
-  extern __shared__ uint32_t sh_bitmap[];
-  __shared__ float sh_f[POP_MAX_THREADS];
-  __shared__ int sh_ext[POP_MAX_THREADS];
-  __shared__ int sh_spk[POP_MAX_THREADS];
+  /* dynamic LDS: three arrays of blockDim entries, then the active-row
+   * bitmap (and, FUSED, the active-row list); see pop_lds() */
+  extern __shared__ uint32_t sh_dyn[];
+  float* const sh_f = reinterpret_cast<float*>(sh_dyn);
+  int* const sh_ext = reinterpret_cast<int*>(sh_dyn + blockDim.x);
+  int* const sh_spk = reinterpret_cast<int*>(sh_dyn + 2 * blockDim.x);
+  uint32_t* const sh_bitmap = sh_dyn + 3 * blockDim.x;
   __shared__ int sh_scan[POP_MAX_THREADS / 64 + 1];
   __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
   __shared__ float sh_vsum;
@@ -807,15 +810,20 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   }
 }
 
+/* six waves per SIMD fit the FP32 and BF16 forms without spills (one wave
+ * more than their natural register count allows); the FP16 form would
+ * spill and keeps five */
+#define RC_WAVES(PREC) ((PREC) == GSBN_PREC_F16 ? 5 : 6)
+
 template <int PREC>
-__global__ void __launch_bounds__(RC_THREADS) rowcol_kernel(gsbn_rowcol_args a) {
+__global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC)) rowcol_kernel(gsbn_rowcol_args a) {
   const int h = blockIdx.x / a.parts;
   const RcStep q = {a.simstep, a.exec, a.parts, a.table_len, a.kp, a.table};
   rowcol_body<PREC>(a, q, h, blockIdx.x - h * a.parts);
 }
 
 template <int PREC>
-__global__ void __launch_bounds__(RC_THREADS)
+__global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC))
     rowcol_kernel_batch(const gsbn_rowcol_args* __restrict__ arr, const int32_t* __restrict__ wg_proj,
                         const int32_t* __restrict__ wg_hcu, int parts, gsbn_step_args st) {
   const int w = blockIdx.x / parts;
@@ -1184,6 +1192,14 @@ static bool trace_form_ok(const A& a) {
   }
 }
 
+/* dynamic LDS of the population update (pop_body): three arrays of
+ * `threads` entries and the active-row bitmap; the fused step adds the
+ * active-row list */
+static size_t pop_lds(int threads, int64_t max_slots, bool fused) {
+  return (size_t)(3 * (int64_t)threads + (max_slots + 31) / 32 + (fused ? max_slots : 0)) *
+         sizeof(uint32_t);
+}
+
 extern "C" {
 
 int gsbn_pop_update(const gsbn_pop_args* a, void* stream) {
@@ -1192,10 +1208,10 @@ int gsbn_pop_update(const gsbn_pop_args* a, void* stream) {
       a->nproj_out < 0 || (a->nproj_out > 0 && !a->outs))
     return GSBN_ERR_ARG;
   if (a->wta_pop_max && !a->scratch_dsup0) return GSBN_ERR_ARG;
-  const size_t lds = (size_t)((a->max_slots + 31) / 32) * sizeof(uint32_t);
-  if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
   hipStream_t s = (hipStream_t)stream;
   const int threads = ((a->M + 63) / 64) * 64;
+  const size_t lds = pop_lds(threads, a->max_slots, false);
+  if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
   if (a->wta_pop_max) {
     hipLaunchKernelGGL(pop_dsup0_kernel, dim3(1), dim3(64), 0, s, *a);
     int rc = launch_status();
@@ -1276,7 +1292,7 @@ int gsbn_pop_update_batch(const gsbn_pop_args* pops_dev, int32_t npop, const int
   if (!pops_dev || !wg_pop || !wg_hcu || !step || nwg <= 0 || npop <= 0 || threads <= 0 ||
       threads > POP_MAX_THREADS || threads % 64 || max_slots < 0)
     return GSBN_ERR_ARG;
-  const size_t lds = (size_t)((max_slots + 31) / 32) * sizeof(uint32_t);
+  const size_t lds = pop_lds(threads, max_slots, false);
   if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
   hipStream_t s = (hipStream_t)stream;
   if (any_wta_pop_max) {
@@ -1318,10 +1334,7 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
   return launch_status();
 }
 
-/* dynamic LDS of the fused step: the active-row bitmap and list */
-static size_t fused_lds(int64_t max_slots) {
-  return (size_t)((max_slots + 31) / 32 + max_slots) * sizeof(uint32_t);
-}
+
 
 int gsbn_step_fused(const gsbn_pop_args* a, const gsbn_rowcol_args* b, void* stream) {
   if (!a || !b || a->H <= 0 || a->M <= 0 || a->M > RC_THREADS || a->nproj_in != 1 || !a->projs ||
@@ -1330,7 +1343,7 @@ int gsbn_step_fused(const gsbn_pop_args* a, const gsbn_rowcol_args* b, void* str
       !trace_form_ok(*b))
     return GSBN_ERR_ARG;
   if (a->wta_pop_max && !a->scratch_dsup0) return GSBN_ERR_ARG;
-  const size_t lds = fused_lds(a->max_slots);
+  const size_t lds = pop_lds(RC_THREADS, a->max_slots, true);
   if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
   hipStream_t s = (hipStream_t)stream;
   if (a->wta_pop_max) {
@@ -1363,7 +1376,7 @@ int gsbn_step_fused_batch(const gsbn_pop_args* pops_dev, int32_t npop,
       step->kp_slot < 0 || step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 ||
       step->prec > GSBN_PREC_F16)
     return GSBN_ERR_ARG;
-  const size_t lds = fused_lds(max_slots);
+  const size_t lds = pop_lds(RC_THREADS, max_slots, true);
   if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
   hipStream_t s = (hipStream_t)stream;
   if (any_wta_pop_max) {

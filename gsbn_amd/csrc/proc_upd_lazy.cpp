@@ -1268,15 +1268,25 @@ void ProcUpdLazy::build_batch_maps() {
     }
     _pop_threads = std::max(_pop_threads, ((p->_dim_mcu + 63) / 64) * 64);
   }
+  /* row/column workgroups of the largest hypercolumns (slots x MCUs, the
+   * size of a column pass and of a row) first: the long ones start early
+   * instead of ending the launch alone */
   _max_slots = 0;
+  vector<int> order(_list_proj.size());
   for (size_t i = 0; i < _list_proj.size(); ++i) {
-    Proj* q = _list_proj[i];
-    for (int h = 0; h < q->_dim_hcu; ++h) {
+    order[i] = (int)i;
+    _max_slots = std::max(_max_slots, _list_proj[i]->_dim_conn);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const Proj* a = _list_proj[x];
+    const Proj* b = _list_proj[y];
+    return (int64_t)a->_dim_conn * a->_dim_mcu > (int64_t)b->_dim_conn * b->_dim_mcu;
+  });
+  for (int i : order)
+    for (int h = 0; h < _list_proj[i]->_dim_hcu; ++h) {
       qp.push_back((int32_t)i);
       qh.push_back(h);
     }
-    _max_slots = std::max(_max_slots, q->_dim_conn);
-  }
   _nwg_pop = (int)wp.size();
   _nhcu_proj = (int)qp.size();
   /* the projection into each population (fused step: exactly one) */
@@ -1343,7 +1353,7 @@ void ProcUpdLazy::update_gpu() {
       _max_slots = 0;
       for (auto* q : _list_proj) _max_slots = std::max(_max_slots, q->_dim_conn);
     }
-    if (((int64_t)(_max_slots + 31) / 32 + _max_slots) * 4 > 48 * 1024) _fusable = false;
+    if ((3 * 256 + (int64_t)(_max_slots + 31) / 32 + _max_slots) * 4 > 48 * 1024) _fusable = false;
     if (_fused_fixed && !_fusable)
       throw Error(-2, "ProcUpdLazy: fused-step needs populations of at most 256 MCUs with one "
                       "incoming projection each, and the slot bitmap and list in 48 KB of LDS");
