@@ -1116,6 +1116,7 @@ class ProcUpdLazy : public ProcedureBase {
   DevBuf<gsbn_rowcol_args> _proj_dev;
   DevBuf<int32_t> _wg_pop, _wg_pop_hcu, _wg_proj, _wg_proj_hcu, _in_proj;
   int _pop_threads = 64, _max_slots = 0, _any_quirk = 0, _nwg_pop = 0, _nhcu_proj = 0, _parts = 1;
+
   int _max_m = 0, _parts0 = 1;
   /* every population has M <= 256 and one incoming projection, and the
    * fused step's LDS (active-row bitmap and list) fits */
@@ -1260,6 +1261,8 @@ void ProcUpdLazy::build_batch_maps() {
   vector<int32_t> wp, wh, qp, qh;
   _pop_threads = 64;
   _any_quirk = _wta_pop_max;
+  /* one launch for all populations, blocks sized for the largest M (one
+   * launch per block size was measured slower on the 512 networks) */
   for (size_t i = 0; i < _list_pop.size(); ++i) {
     Pop* p = _list_pop[i];
     for (int h = 0; h < p->_dim_hcu; ++h) {

@@ -118,3 +118,16 @@ def test_batched_networks_bitwise(tmp_path):
     c, t, flags = gpu.check()
     assert (c, t) == (ora.check.correct, ora.check.total)
     gpu.close()
+
+
+def test_batched_networks_mixed_block_sizes(tmp_path):
+    # populations of 40-140 MCUs in one batched population launch of
+    # 192-thread blocks (most threads idle for the small ones), the
+    # row/column workgroups largest hypercolumn first
+    spec = configs.batched_nets(6, hcu_range=(2, 4), mcu_range=(40, 140), n_pat=3, train=30,
+                                quiet=5, recall=20, reset=10)
+    assert len({(p.mcu + 63) // 64 for p in spec.pops}) > 1
+    gpu, ora = _lockstep_and_compare(spec, tmp_path)
+    c, t, flags = gpu.check()
+    assert (c, t) == (ora.check.correct, ora.check.total)
+    gpu.close()
