@@ -44,13 +44,13 @@
 #define RE_THREADS_MAX 1024
 #define RE_RING 64 /* spike-history slots; latency di + 2 <= 33 */
 
-/* GEMM tiling: 128 x 128 output tile per workgroup of 4 waves (64 x 64 each,
- * 2 x 2 blocks of 32 x 32), K in steps of 64, LDS rows padded by 8 bf16 */
-#define GM_BM 128
-#define GM_BN 128
+/* GEMM tiling: 256 x 256 output tile per workgroup of 16 waves (64 x 64
+ * each, 2 x 2 blocks of 32 x 32), K in steps of 64, LDS rows padded by 8 bf16 */
+#define GM_BM 256
+#define GM_BN 256
 #define GM_BK 64
 #define GM_LDK (GM_BK + 8)
-#define GM_THREADS 256
+#define GM_THREADS 1024
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 gsbn_bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float gsbn_f32x16;
@@ -211,21 +211,28 @@ __global__ void __launch_bounds__(RE_THREADS_MAX) recall_step_kernel(ReArgs a, i
  * accumulation on v_mfma_f32_32x32x16_bf16.  S and WT have Kp columns (Kp a
  * multiple of GM_BK, zero padded); S has ceil(B / GM_BM) * GM_BM rows and WT
  * ceil(N / GM_BN) * GM_BN rows, so tiles never read past them.
+ * A 256 x 256 output tile per workgroup of 16 waves (4 x 4, 64 x 64 each:
+ * 2 x 2 MFMA blocks), K in steps of 64 through one LDS buffer (rows padded
+ * by 8 bf16 against bank conflicts) with the next K tile staged in
+ * registers during the current one's MFMAs.  (Measured at this problem's
+ * shape: 128 x 128 tiles of 4 waves reach 0.79 PFLOP/s, double-buffered LDS
+ * 0.75, 256 x 256 tiles of 16 waves 0.96; larger wave tiles, 8-wave
+ * workgroups and K steps of 32 do worse.)
  * Workgroup w is remapped so that consecutive tiles of one XCD (hardware
  * workgroup w runs on XCD w % 8) share their WT tile in that XCD's L2. */
 __global__ void __launch_bounds__(GM_THREADS)
     spike_gemm_kernel(const uint16_t* __restrict__ S, const uint16_t* __restrict__ WT,
                       float* __restrict__ E, int Bn, int N, int Kp, float beta, int mtiles,
                       int ntiles) {
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][GM_BM * GM_LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][GM_BN * GM_LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t sA[GM_BM * GM_LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[GM_BN * GM_LDK];
   const int nwg = mtiles * ntiles;
   int wg = blockIdx.x;
   if ((nwg & 7) == 0) wg = (wg & 7) * (nwg >> 3) + (wg >> 3);
   const int nt = wg / mtiles, mt = wg - nt * mtiles;
   const int m0 = mt * GM_BM, n0 = nt * GM_BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave >> 2, wn = wave & 3;
 
   gsbn_f32x16 acc[2][2];
 #pragma unroll
@@ -235,39 +242,31 @@ __global__ void __launch_bounds__(GM_THREADS)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][jj][r] = 0.0f;
 
-  /* global -> register staging: 4 chunks of 16 bytes per thread and matrix;
-   * chunk c = tid + 256 i covers row c / 8, columns (c % 8) * 8 .. +7 of the
-   * K tile.  Named registers (no arrays) keep the staging out of scratch. */
-  const int crow = tid >> 3, ccol = (tid & 7) * 8; /* chunk i: row crow + 32 i */
+  /* global -> register staging: 2 chunks of 16 bytes per thread and matrix;
+   * chunk c = tid + 1024 i covers row c / 8, columns (c % 8) * 8 .. +7 of
+   * the K tile.  Named registers (no arrays) keep the staging out of
+   * scratch. */
+  const int crow = tid >> 3, ccol = (tid & 7) * 8; /* chunk i: row crow + 128 i */
   const uint16_t* pa = S + (int64_t)(m0 + crow) * Kp + ccol;
   const uint16_t* pb = WT + (int64_t)(n0 + crow) * Kp + ccol;
-  const int64_t rs = (int64_t)32 * Kp; /* 32 rows further */
-  uint4 a0, a1, a2, a3, b0, b1, b2, b3;
-#define GM_LOAD(k0)                                                 \
-  a0 = *reinterpret_cast<const uint4*>(pa + (k0));                  \
-  a1 = *reinterpret_cast<const uint4*>(pa + rs + (k0));             \
-  a2 = *reinterpret_cast<const uint4*>(pa + 2 * rs + (k0));         \
-  a3 = *reinterpret_cast<const uint4*>(pa + 3 * rs + (k0));         \
-  b0 = *reinterpret_cast<const uint4*>(pb + (k0));                  \
-  b1 = *reinterpret_cast<const uint4*>(pb + rs + (k0));             \
-  b2 = *reinterpret_cast<const uint4*>(pb + 2 * rs + (k0));         \
-  b3 = *reinterpret_cast<const uint4*>(pb + 3 * rs + (k0));
-#define GM_STORE(buf)                                                                  \
-  *reinterpret_cast<uint4*>(&sA[buf][(crow)*GM_LDK + ccol]) = a0;                      \
-  *reinterpret_cast<uint4*>(&sA[buf][(crow + 32) * GM_LDK + ccol]) = a1;               \
-  *reinterpret_cast<uint4*>(&sA[buf][(crow + 64) * GM_LDK + ccol]) = a2;               \
-  *reinterpret_cast<uint4*>(&sA[buf][(crow + 96) * GM_LDK + ccol]) = a3;               \
-  *reinterpret_cast<uint4*>(&sB[buf][(crow)*GM_LDK + ccol]) = b0;                      \
-  *reinterpret_cast<uint4*>(&sB[buf][(crow + 32) * GM_LDK + ccol]) = b1;               \
-  *reinterpret_cast<uint4*>(&sB[buf][(crow + 64) * GM_LDK + ccol]) = b2;               \
-  *reinterpret_cast<uint4*>(&sB[buf][(crow + 96) * GM_LDK + ccol]) = b3;
+  const int64_t rs = (int64_t)128 * Kp; /* 128 rows further */
+  uint4 a0, a1, b0, b1;
+#define GM_LOAD(k0)                                     \
+  a0 = *reinterpret_cast<const uint4*>(pa + (k0));      \
+  a1 = *reinterpret_cast<const uint4*>(pa + rs + (k0)); \
+  b0 = *reinterpret_cast<const uint4*>(pb + (k0));      \
+  b1 = *reinterpret_cast<const uint4*>(pb + rs + (k0));
+#define GM_STORE()                                                           \
+  *reinterpret_cast<uint4*>(&sA[(crow)*GM_LDK + ccol]) = a0;                 \
+  *reinterpret_cast<uint4*>(&sA[(crow + 128) * GM_LDK + ccol]) = a1;         \
+  *reinterpret_cast<uint4*>(&sB[(crow)*GM_LDK + ccol]) = b0;                 \
+  *reinterpret_cast<uint4*>(&sB[(crow + 128) * GM_LDK + ccol]) = b1;
 
   const int nk = Kp / GM_BK;
   GM_LOAD(0)
-  GM_STORE(0)
+  GM_STORE()
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
     if (kt + 1 < nk) {
       GM_LOAD((kt + 1) * GM_BK)
     }
@@ -278,11 +277,11 @@ __global__ void __launch_bounds__(GM_THREADS)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
         af[i] = *reinterpret_cast<const gsbn_bf16x8*>(
-            &sA[buf][(wm * 64 + i * 32 + (lane & 31)) * GM_LDK + kk]);
+            &sA[(wm * 64 + i * 32 + (lane & 31)) * GM_LDK + kk]);
 #pragma unroll
       for (int jj = 0; jj < 2; ++jj)
         bf[jj] = *reinterpret_cast<const gsbn_bf16x8*>(
-            &sB[buf][(wn * 64 + jj * 32 + (lane & 31)) * GM_LDK + kk]);
+            &sB[(wn * 64 + jj * 32 + (lane & 31)) * GM_LDK + kk]);
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -290,9 +289,10 @@ __global__ void __launch_bounds__(GM_THREADS)
           acc[i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bf[jj], acc[i][jj], 0, 0, 0);
     }
     if (kt + 1 < nk) {
-      GM_STORE(buf ^ 1)
+      __syncthreads(); /* every wave is done with this K tile */
+      GM_STORE()
+      __syncthreads();
     }
-    __syncthreads();
   }
 #undef GM_LOAD
 #undef GM_STORE

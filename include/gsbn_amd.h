@@ -407,8 +407,8 @@ int gsbn_recall_run(gsbn_recall* r, const int16_t* clamp, const float* wmask, in
                     int32_t count_from, int32_t* out_idx, int32_t* out_cnt, int8_t* raster,
                     double* support_ms);
 /* E[b][n] = beta E[b][n] + sum_k S[b][k] WT[n][k], S and WT bfloat16 with
- * Kp columns (Kp % 64 == 0, zero padded), S with ceil(B/128)*128 rows and WT
- * with ceil(N/128)*128 rows (padding rows zero), E float [B][N]; device
+ * Kp columns (Kp % 64 == 0, zero padded), S with ceil(B/256)*256 rows and WT
+ * with ceil(N/256)*256 rows (padding rows zero), E float [B][N]; device
  * pointers. */
 int gsbn_spike_gemm(int32_t B, int32_t N, int32_t Kp, const uint16_t* S, const uint16_t* WT,
                     float* E, float beta, void* stream);

@@ -29,8 +29,8 @@ def _gemm(S, WT, E, beta):
     B, Kp = S.shape
     N = E.shape[1]
     dev = torch.device("cuda:0")
-    Bp = (B + 127) // 128 * 128
-    Np = (N + 127) // 128 * 128
+    Bp = (B + 255) // 256 * 256
+    Np = (N + 255) // 256 * 256
     s = torch.zeros((Bp, Kp), dtype=torch.int16, device=dev)
     w = torch.zeros((Np, Kp), dtype=torch.int16, device=dev)
     s[:B] = torch.from_numpy(S.view(np.int16)).to(dev)
