@@ -131,6 +131,11 @@ class SyncVector : public VecBase {
  public:
   explicit SyncVector(const string& n) : VecBase(n, dtype_of<T>::v) {}
   ~SyncVector() override {
+    if (stage_ev_) {
+      (void)hipEventSynchronize(stage_ev_);
+      (void)hipEventDestroy(stage_ev_);
+    }
+    if (stage_) (void)hipHostFree(stage_);
     if (d_) (void)hipFree(d_);
   }
   size_t size() const override { return n_; }
@@ -231,9 +236,7 @@ class SyncVector : public VecBase {
     if (status == HOST_NEWER) {
       if (n_) {
         if (h_.size() < n_) h_.resize(n_);
-        hip_check(hipMemcpyAsync(d_, h_.data(), n_ * sizeof(T), hipMemcpyHostToDevice, stream()),
-                  "H2D");
-        hip_check(hipStreamSynchronize(stream()), "sync");
+        upload();
       }
       status = SYNCED;
     }
@@ -241,6 +244,38 @@ class SyncVector : public VecBase {
   vector<T>& raw_host() { return h_; }
 
  private:
+  /* host -> device copy in stream order.  Vectors of up to kStageMax bytes
+   * (stimulus rows, masks, counters) go through a pinned staging buffer of
+   * their own, so the call returns without waiting for the device work
+   * queued before it; the buffer is refilled only after its previous copy
+   * has run (an event).  Larger vectors wait for their copy. */
+  static constexpr size_t kStageMax = (size_t)4 << 20;
+  void upload() {
+    const size_t bytes = n_ * sizeof(T);
+    if (bytes <= kStageMax) {
+      if (stage_ev_)
+        hip_check(hipEventSynchronize(stage_ev_), "staging event");
+      else
+        hip_check(hipEventCreateWithFlags(&stage_ev_, hipEventDisableTiming), "hipEventCreate");
+      if (stage_n_ < bytes) {
+        if (stage_) hip_check(hipHostFree(stage_), "hipHostFree");
+        stage_ = nullptr;
+        stage_n_ = 0;
+        hip_check(hipHostMalloc(&stage_, bytes, hipHostMallocDefault), "hipHostMalloc");
+        stage_n_ = bytes;
+      }
+      memcpy(stage_, h_.data(), bytes);
+      hip_check(hipMemcpyAsync(d_, stage_, bytes, hipMemcpyHostToDevice, stream()), "H2D");
+      hip_check(hipEventRecord(stage_ev_, stream()), "hipEventRecord");
+      return;
+    }
+    hip_check(hipMemcpyAsync(d_, h_.data(), bytes, hipMemcpyHostToDevice, stream()), "H2D");
+    hip_check(hipStreamSynchronize(stream()), "sync");
+  }
+  void* stage_ = nullptr;
+  size_t stage_n_ = 0;
+  hipEvent_t stage_ev_ = nullptr;
+
   size_t offset(int i) const {
     if (i <= 0) return 0;
     return ld_ <= 0 ? (size_t)i : (size_t)i * (size_t)ld_;

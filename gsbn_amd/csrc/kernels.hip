@@ -205,13 +205,20 @@ struct FusedLds {
 };
 
 /* One hypercolumn's population update.  FUSED: `f` also receives what the
- * fused step's row/column phase needs (one incoming projection). */
+ * fused step's row/column phase needs (one incoming projection).
+ *
+ * Memory order: every load whose address does not depend on this step's
+ * results is issued in one batch at the start (the first projection's
+ * pointers come from the arguments, and its first four pending support
+ * contributions are read before their count is known), and every global
+ * store of the update is issued after the last load: loads and stores share
+ * one in-order counter, so a load issued behind stores would wait for them. */
 template <bool FUSED>
 __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
                                          FusedLds* f = nullptr) {
   /* dynamic LDS: three arrays of blockDim entries, then the active-row
    * bitmap (and, FUSED, the active-row list); see pop_lds() */
-  extern __shared__ uint32_t sh_dyn[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t sh_dyn[];
   float* const sh_f = reinterpret_cast<float*>(sh_dyn);
   int* const sh_ext = reinterpret_cast<int*>(sh_dyn + blockDim.x);
   int* const sh_spk = reinterpret_cast<int*>(sh_dyn + 2 * blockDim.x);
@@ -228,30 +235,29 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   const int idx = h * M + j;
   const int64_t N = (int64_t)a.H * M;
   const int nin = a.nproj_in;
+  const int bd = blockDim.x;
 
-  /* 0. every load that does not depend on this step's results is issued
-   * here, so that their latencies overlap (the first incoming projection's
-   * state and ring word; further projections load where they are used) */
-  float dsup_old = 0.0f, lg = 0.0f, wmv = 0.0f, e0 = 0.0f, bj0 = 0.0f, pj0 = 0.0f, ej0 = 0.0f,
-        zj0 = 0.0f;
+  /* 0. loads that do not depend on this step's results */
+  float dsup_old = 0.0f, lg = 0.0f, e0 = 0.0f, bj0 = 0.0f, pj0 = 0.0f, ej0 = 0.0f, zj0 = 0.0f;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f; /* first pending contributions */
   int32_t counter_old = 0;
   uint32_t fut0 = 0u, fut1 = 0u, fut2 = 0u, fut3 = 0u; /* words j + i * blockDim of the ring slot */
   int pend0 = 0;
+  const float wmv = a.wmask[h];
   if (live) {
     dsup_old = a.dsup[idx];
     lg = a.lginp[idx];
-    wmv = a.wmask[h];
     counter_old = a.counter[idx];
   }
   /* fan-out range of this unit in the first outgoing projection (the queue
    * push of step 5b) */
   int fo0 = 0, fo1 = 0;
   if (a.nproj_out > 0 && live) {
-    fo0 = a.outs[0].fwd_off[idx];
-    fo1 = a.outs[0].fwd_off[idx + 1];
+    fo0 = a.out0.fwd_off[idx];
+    fo1 = a.out0.fwd_off[idx + 1];
   }
   if (nin > 0) {
-    const gsbn_pop_proj& p0 = a.projs[0];
+    const gsbn_pop_proj& p0 = a.proj0;
     pend0 = p0.pend_cnt[h];
     if (live) {
       e0 = a.epsc[idx];
@@ -259,13 +265,26 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
       pj0 = p0.pj[idx];
       ej0 = p0.ej[idx];
       zj0 = p0.zj[idx];
+      /* rows 0..3 of the hypercolumn's contribution block (inside the block
+       * whatever the count; entries at or beyond the count are not used) */
+      const float* c = p0.contrib + ((int64_t)h * p0.C) * M + j;
+      if (p0.C > 0) c0 = c[0];
+      if (p0.C > 1) c1 = c[M];
+      if (p0.C > 2) c2 = c[2 * M];
+      if (p0.C > 3) c3 = c[3 * M];
     }
     const uint32_t* wslot = p0.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * p0.W;
-    const int bd = blockDim.x;
     if (j < p0.W) fut0 = wslot[j];
     if (j + bd < p0.W) fut1 = wslot[j + bd];
     if (j + 2 * bd < p0.W) fut2 = wslot[j + 2 * bd];
     if (j + 3 * bd < p0.W) fut3 = wslot[j + 3 * bd];
+  }
+  const float d0 = a.wta_pop_max ? a.scratch_dsup0[0] : 0.0f;
+  const bool ext_on = a.ext_off && a.simstep >= 0 && a.simstep < a.ext_nsteps;
+  int ext_b = 0, ext_e = 0;
+  if (ext_on) {
+    ext_b = a.ext_off[a.simstep];
+    ext_e = a.ext_off[a.simstep + 1];
   }
   /* the draws and the first projection's postsynaptic traces do not depend
    * on this step's results either: computed while the loads are in flight */
@@ -280,7 +299,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   if (a.do_jxx && live && nin > 0) {
     /* update_jxx_kernel_cpu (Proj.cpp:310-363) for projection 0; the spike
      * increment of zj follows in step 6 */
-    const gsbn_pop_proj& p0 = a.projs[0];
+    const gsbn_pop_proj& p0 = a.proj0;
     const float kp = p0.taupdt * a.prn;
     pj_n = pj0 + (ej0 - pj0) * kp;
     ej_n = ej0 + (zj0 - ej0) * p0.ke;
@@ -290,20 +309,31 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   }
   if (j > 0 && j < nin) sh_pend[j] = a.projs[j].pend_cnt[h];
   __syncthreads();
+  /* the first external spike of each thread (its address needs ext_b) */
+  int ext_u0 = -1;
+  if (ext_on && ext_b + j < ext_e) ext_u0 = a.ext_units[ext_b + j];
 
   /* 1. pending support contributions of the previous step's row updates,
-   * added in ascending row order; four loads in flight at a time */
+   * added in ascending row order */
   float efin[GSBN_MAX_PROJ_IN];
   for (int m = 0; m < nin; ++m) {
     if (!live) {
       efin[m] = 0.0f;
       continue;
     }
-    const gsbn_pop_proj& p = a.projs[m];
+    const gsbn_pop_proj& p = m == 0 ? a.proj0 : a.projs[m];
     const int cnt = m == 0 ? pend0 : sh_pend[m];
     float e = m == 0 ? e0 : a.epsc[m * N + idx];
     const float* c = p.contrib + ((int64_t)h * p.C) * M + j;
-    for (int k = 0; k < cnt; k += 4) {
+    int k = 0;
+    if (m == 0) {
+      const float pre[4] = {c0, c1, c2, c3};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q < cnt && gsbn_fbits(pre[q]) != GSBN_NO_CONTRIB_BITS) e = e + pre[q];
+      k = 4;
+    }
+    for (; k < cnt; k += 4) {
       float v[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = k + q < cnt ? c[(int64_t)(k + q) * M] : 0.0f;
@@ -312,7 +342,6 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
         if (k + q < cnt && gsbn_fbits(v[q]) != GSBN_NO_CONTRIB_BITS) e = e + v[q];
     }
     efin[m] = e;
-    a.epsc[m * N + idx] = e;
   }
 
   /* 2. support, dsup (update_sup_kernel_1_cpu, Pop.cpp:229-261) */
@@ -323,9 +352,6 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     float sup = a.lgbias + a.igain * lg + xi;
     sup += (a.wgain * wmv) * wsup;
     dsup = dsup_old + (sup - dsup_old) * a.taumdt;
-    a.dsup[idx] = dsup;
-    a.rnd_u[idx] = u;
-    a.rnd_n[idx] = xi;
     sh_f[j] = dsup;
   }
   __syncthreads();
@@ -351,14 +377,14 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     if (o > maxdsup) maxdsup = o;
   }
   if (a.wta_pop_max) {
-    float d0 = a.scratch_dsup0[0];
     float mx = d0;
     if (maxdsup > mx) mx = maxdsup;
     maxdsup = mx;
   }
   __syncthreads();
 
-  /* soft-max (Pop.cpp:280-302) */
+  /* soft-max (Pop.cpp:280-302); the normaliser is summed in ascending unit
+   * order by one thread, four LDS words per read */
   const float maxact = gsbn_expf(a.wtagain * maxdsup);
   float act = 0.0f;
   if (live) {
@@ -369,7 +395,17 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   __syncthreads();
   if (j == 0) {
     float vsum = 0.0f;
-    for (int m = 0; m < M; ++m) vsum += sh_f[m];
+    const float4* v4 = reinterpret_cast<const float4*>(sh_f);
+    int m = 0;
+#pragma unroll 5
+    for (; m + 4 <= M; m += 4) {
+      const float4 x = v4[m >> 2];
+      vsum += x.x;
+      vsum += x.y;
+      vsum += x.z;
+      vsum += x.w;
+    }
+    for (; m < M; ++m) vsum += sh_f[m];
     sh_vsum = vsum;
   }
   __syncthreads();
@@ -378,56 +414,77 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
 
   /* 4. spikes and counters (Pop.cpp:305-323); external override (Pop.cpp:389-409) */
   int spk = 0;
-  if (live) {
-    a.act[idx] = act;
-    spk = (u < act * a.maxfqdt) ? 1 : 0;
-    a.counter[idx] = counter_old + spk;
-  }
+  if (live) spk = (u < act * a.maxfqdt) ? 1 : 0;
+  const int32_t counter_new = counter_old + spk;
   if (a.ext_off) {
     sh_ext[j] = 0;
     __syncthreads();
-    if (a.simstep >= 0 && a.simstep < a.ext_nsteps) {
-      const int b = a.ext_off[a.simstep], e = a.ext_off[a.simstep + 1];
-      for (int t = b + j; t < e; t += blockDim.x) {
-        int unit = a.ext_units[t];
-        if (unit >= h * M && unit < h * M + M) sh_ext[unit - h * M] = 1;
-      }
+    for (int t = ext_b + j; t < ext_e; t += bd) {
+      const int unit = t == ext_b + j ? ext_u0 : a.ext_units[t];
+      if (unit >= h * M && unit < h * M + M) sh_ext[unit - h * M] = 1;
     }
     __syncthreads();
     if (live) spk = sh_ext[j];
   }
-  if (live) a.spike[idx] = (int8_t)spk;
   if (FUSED && live) sh_sj[j] = spk;
 
   /* 5. spiking units of this hypercolumn, ascending (.ssj, Proj.cpp:632-643) */
   int spk_tot;
+  int pos_s = 0;
   {
-    int pos = block_scan_fast(live ? spk : 0, sh_scan, &spk_tot);
+    pos_s = block_scan_fast(live ? spk : 0, sh_scan, &spk_tot);
     if (live && spk) {
-      a.spk_list[h * M + pos] = j;
-      sh_spk[pos] = j;
-      sh_ext[pos] = fo0; /* the fan-out range of the first outgoing projection */
-      sh_f[pos] = __int_as_float(fo1);
+      sh_spk[pos_s] = j;
+      sh_ext[pos_s] = fo0; /* the fan-out range of the first outgoing projection */
+      sh_f[pos_s] = __int_as_float(fo1);
     }
-    if (j == 0) a.spk_cnt[h] = spk_tot;
   }
+  if (FUSED && live) sh_pj[j] = a.do_jxx ? pj_n : pj0;
+
   /* 5b. queue push of this step's spikes (what the queue update of the next
-   * step would push from si, Proj.cpp:645-664) into the outgoing rings */
+   * step would push from si, Proj.cpp:645-664) into the outgoing rings; its
+   * fan-out reads come before the stores below */
   if (a.nproj_out > 0 && spk_tot > 0) {
     __syncthreads();
     for (int i = 0; i < spk_tot; ++i)
-      push_rows(a.outs[0], sh_ext[i], __float_as_int(sh_f[i]), a.exec, threadIdx.x, blockDim.x);
+      push_rows(a.out0, sh_ext[i], __float_as_int(sh_f[i]), a.exec, threadIdx.x, blockDim.x);
     for (int o = 1; o < a.nproj_out; ++o)
       for (int i = 0; i < spk_tot; ++i)
         push_unit(a.outs[o], h * M + sh_spk[i], a.exec, threadIdx.x, blockDim.x);
   }
 
+  /* 7a. the first projection's ring slot of this exec step into LDS (from
+   * the words read at the start; a slot of more than four words per thread
+   * reads the rest here, before the stores) */
+  if (nin > 0) {
+    const int W = a.proj0.W;
+    const uint32_t* words = a.proj0.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * W;
+    for (int w = j, i = 0; w < W; w += bd, ++i)
+      sh_bitmap[w] = i == 0 ? fut0 : i == 1 ? fut1 : i == 2 ? fut2 : i == 3 ? fut3 : words[w];
+  }
+
+  /* --- stores: the population's state (steps 1-4) --- */
+  if (live) {
+    a.dsup[idx] = dsup;
+    a.rnd_u[idx] = u;
+    a.rnd_n[idx] = xi;
+    a.act[idx] = act;
+    a.counter[idx] = counter_new;
+    a.spike[idx] = (int8_t)spk;
+    if (spk) a.spk_list[h * M + pos_s] = j;
+  }
+  if (j == 0) a.spk_cnt[h] = spk_tot;
+
   /* 6. postsynaptic traces and bias for every incoming projection
-   * (update_jxx_kernel_cpu, Proj.cpp:310-363) */
-  if (FUSED && live) sh_pj[j] = a.do_jxx ? pj_n : pj0;
-  if (a.do_jxx && live) {
+   * (update_jxx_kernel_cpu, Proj.cpp:310-363); epsc keeps the finalised sum
+   * when the traces are not updated */
+  if (live) {
     for (int m = 0; m < nin; ++m) {
-      const gsbn_pop_proj& p = a.projs[m];
+      const gsbn_pop_proj& p = m == 0 ? a.proj0 : a.projs[m];
+      if (!a.do_jxx) {
+        a.epsc[m * N + idx] = efin[m];
+        continue;
+      }
       a.epsc[m * N + idx] = efin[m] * (1 - p.kepsc);
       if (m == 0) {
         if (spk > 0) zj_n += p.kftj;
@@ -451,22 +508,18 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   }
 
   /* 7. active rows of this step (Proj.cpp:621-630): the ring slot of this
-   * exec step, read into LDS and cleared for its next use */
+   * exec step, listed in ascending order and cleared for its next use */
   for (int m = 0; m < a.nproj_in; ++m) {
-    const gsbn_pop_proj& p = a.projs[m];
+    const gsbn_pop_proj& p = m == 0 ? a.proj0 : a.projs[m];
     const int C = p.C;
     const int W = p.W;
     uint32_t* words = p.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * W;
-    for (int w = j, i = 0; w < W; w += blockDim.x, ++i) {
-      uint32_t v;
-      if (m == 0 && i < 4)
-        v = i == 0 ? fut0 : i == 1 ? fut1 : i == 2 ? fut2 : fut3;
-      else
-        v = words[w];
-      sh_bitmap[w] = v;
-      if (v) words[w] = 0u;
+    if (m > 0) {
+      for (int w = j; w < W; w += bd) sh_bitmap[w] = words[w];
     }
     __syncthreads();
+    for (int w = j; w < W; w += bd)
+      if (sh_bitmap[w]) words[w] = 0u;
     const int per = (W + blockDim.x - 1) / blockDim.x;
     const int w0 = j * per;
     const int w1 = (w0 + per < W) ? w0 + per : W;
@@ -685,10 +738,23 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   const int t = q.t;
   const bool learn = q.kp != 0.0f;
 
-  /* rows part, part+P, ... of this HCU's ascending active-row list */
-  const int nrows = FUSED ? fl->nrows : a.rows_cnt[h];
-  const int mine = nrows > part ? (nrows - part + P - 1) / P : 0;
+  /* rows part, part+P, ... of this HCU's ascending active-row list.  The
+   * counts, this thread's first row and the first spiking unit are read in
+   * one batch, before either is known to be needed (all inside their
+   * arrays). */
   const int rpt = M >= RC_THREADS ? 1 : RC_THREADS / M;
+  int nrows, nspk, k_pre = 0, j_pre = 0;
+  if (FUSED) {
+    nrows = fl->nrows;
+    nspk = fl->nspk;
+  } else {
+    nrows = a.rows_cnt[h];
+    nspk = a.spk_cnt[h];
+    const int aa0 = part + P * (tid / M);
+    if (tid / M < rpt && aa0 < C) k_pre = a.rows_list[(int64_t)h * C + aa0];
+    j_pre = a.spk_list[h * M];
+  }
+  const int mine = nrows > part ? (nrows - part + P - 1) / P : 0;
   for (int base = 0; base < mine; base += rpt) {
     const int nr = (mine - base < rpt) ? mine - base : rpt;
     /* every thread of a row loads the row record with its synapse record and
@@ -701,7 +767,8 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
       const int rr = f / M;
       const int j = f - rr * M;
       const int aa = part + P * (base + rr);
-      const int k = FUSED ? fl->rows[aa] : a.rows_list[(int64_t)h * C + aa];
+      const int k = FUSED ? fl->rows[aa] : (base == 0 && f == tid) ? k_pre
+                                                                   : a.rows_list[(int64_t)h * C + aa];
       const int64_t r = (int64_t)h * C + k;
       const int64_t sidx = r * M + j;
       gsbn_row_rec st = a.rowst[r];
@@ -748,7 +815,6 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   }
 
   /* slice `part` of the slots of every spiking unit of this HCU */
-  const int nspk = FUSED ? fl->nspk : a.spk_cnt[h];
   if (nspk == 0) return;
   const int slice = (C + P - 1) / P;
   const int k0 = part * slice;
@@ -758,7 +824,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
    * coefficients, then the updates */
   constexpr int U = RC_COLS;
   for (int s_i = 0; s_i < nspk; ++s_i) {
-    const int j = FUSED ? fl->spk[s_i] : a.spk_list[h * M + s_i];
+    const int j = FUSED ? fl->spk[s_i] : s_i == 0 ? j_pre : a.spk_list[h * M + s_i];
     for (int kb = k0 + tid; kb < k1; kb += U * RC_THREADS) {
       bool ok[U];
       int64_t sidx[U];
@@ -1260,13 +1326,39 @@ int gsbn_proj_jxx(const gsbn_jxx_args* a, void* stream) {
   return launch_status();
 }
 
+int gsbn_rowcol_resident(int32_t prec, int32_t* out) {
+  if (!out) return GSBN_ERR_ARG;
+  int dev = 0, cus = 0, per = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) {
+    switch (prec) {
+      case GSBN_PREC_F32:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, rowcol_kernel<GSBN_PREC_F32>, RC_THREADS, 0);
+        break;
+      case GSBN_PREC_BF16:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, rowcol_kernel<GSBN_PREC_BF16>, RC_THREADS, 0);
+        break;
+      default:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, rowcol_kernel<GSBN_PREC_F16>, RC_THREADS, 0);
+        break;
+    }
+  }
+  if (e != hipSuccess) return ret(e);
+  *out = per * cus;
+  return 0;
+}
+
 int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream) {
   if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2 || a->parts < 0 ||
       !trace_form_ok(*a))
     return GSBN_ERR_ARG;
   gsbn_rowcol_args b = *a;
   if (b.parts == 0) {
-    int p = 1536 / b.H;
+    int32_t res = 0;
+    const int rc = gsbn_rowcol_resident(b.prec, &res);
+    if (rc) return rc;
+    const int p = res / b.H;
     b.parts = p < 1 ? 1 : (p > 32 ? 32 : p);
   }
   const int64_t grid = (int64_t)b.H * b.parts;

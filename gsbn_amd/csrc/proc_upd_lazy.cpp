@@ -538,6 +538,7 @@ gsbn_pop_args Pop::args(int simstep, int exec, float prn, int lginp_idx, int wma
     }
     a.nproj_out = (int32_t)outs.size();
     a.outs = outs_dev.p;
+    a.out0 = outs[0];
   }
   a.nproj_in = (int)_in.size();
   a.do_jxx = do_jxx ? 1 : 0;
@@ -554,6 +555,7 @@ gsbn_pop_args Pop::args(int simstep, int exec, float prn, int lginp_idx, int wma
     projs_dev.upload(descs.data(), descs.size());
   }
   a.projs = projs_dev.p;
+  if (!descs.empty()) a.proj0 = descs[0];
   if (!work.p) {
     work.alloc(2 * GSBN_MAX_PROJ_IN);
     vector<int64_t> z(2 * GSBN_MAX_PROJ_IN, 0);
@@ -1158,6 +1160,15 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
   if (_cand.empty()) {
     for (int c : {1, 2, 4, 8, 16, 32})
       if ((int64_t)c * std::max(_nhcu_proj, 1) <= 262144) _cand.push_back(c);
+    /* and the largest split whose grid the device holds at once (a grid
+     * beyond it waits for a second dispatch round) */
+    int32_t res = 0;
+    const int prec = _list_proj.empty() ? GSBN_PREC_F32 : _list_proj[0]->prec;
+    if (gsbn_rowcol_resident(prec, &res) == 0) {
+      const int fit = res / std::max(_nhcu_proj, 1);
+      if (fit > 1 && fit < 32 && std::find(_cand.begin(), _cand.end(), fit) == _cand.end())
+        _cand.push_back(fit);
+    }
     if (_cand.empty()) _cand.push_back(1);
     if (_fusable) _cand.push_back(-1);
     _tev.resize(2 * _cand.size() * kTuneSamples);

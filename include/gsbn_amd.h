@@ -156,6 +156,11 @@ typedef struct {
   int32_t max_slots; /* largest C among the incoming projections (sizes the row bitmap) */
   const gsbn_pop_proj* projs; /* device array, index = position of the projection in the population */
   int64_t* work; /* optional device counters [2*nproj_in]: row and column synapse updates */
+  /* copies of projs[0] (when nproj_in > 0) and outs[0] (when nproj_out > 0):
+   * the kernel takes the first projection's pointers from its arguments, so
+   * its loads do not wait for a read of the device arrays */
+  gsbn_pop_proj proj0;
+  gsbn_fanout out0;
 } gsbn_pop_args;
 int gsbn_pop_update(const gsbn_pop_args* a, void* stream);
 
@@ -230,6 +235,12 @@ typedef struct {
   float p_scale, e_scale; /* GSBN_PREC_F16 only; powers of two */
 } gsbn_rowcol_args;
 int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream);
+
+/* Row/column workgroups of trace form `prec` that the current device holds
+ * at once (resident workgroups per CU times CUs): a split whose grid
+ * (HCUs x parts) fits starts in one dispatch round.  parts == 0 in
+ * gsbn_proj_rowcol picks the largest such split (at most 32). */
+int gsbn_rowcol_resident(int32_t prec, int32_t* out);
 
 /* Batched forms: many populations / projections (e.g. independent networks)
  * in one launch.  The argument structs live in device memory (their per-step
