@@ -131,6 +131,10 @@ class NetSpec:
     # launches' work split; both 0: chosen online
     fused_step: int = 0
     rowcol_parts: int = 0
+    # chained_step 1: one launch per step that runs the previous step's
+    # row/column updates with this step's population update; -1 never;
+    # 0: one of the plans chosen online
+    chained_step: int = 0
 
     @property
     def total_hcu(self) -> int:
@@ -289,6 +293,8 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('  argi { key: "fused-step" val: 1 }')
     if spec.rowcol_parts:
         L.append('  argi { key: "rowcol-parts" val: %d }' % spec.rowcol_parts)
+    if spec.chained_step:
+        L.append('  argi { key: "chained-step" val: %d }' % spec.chained_step)
     L.append("}")
     if spec.conn_arrays is None:
         L.append("proc_param {")

@@ -72,6 +72,7 @@ int gsbn_solver_snapshot(gsbn_solver* s, const char* path) {
     glv.geti("simstep", simstep);
     glv.getf("dt", dt);
     glv.getf("prn", prn);
+    gsbn::active_stream() = s->s->stream();
     s->s->db().save_state(path, simstep * dt, prn);
     return 0;
   });
@@ -202,6 +203,7 @@ int gsbn_solver_sync(gsbn_solver* s) {
 int64_t gsbn_solver_vector_info(gsbn_solver* s, const char* name, int* dtype) {
   try {
     if (!s || !s->s || !name) throw Error(GSBN_ERR_ARG, "null argument");
+    gsbn::active_stream() = s->s->stream(); /* host copies sync on this solver's stream */
     gsbn::VecBase* v = s->s->db().find(name);
     if (!v) throw Error(GSBN_ERR_NAME, std::string("no vector named ") + name);
     if (dynamic_size(name)) (void)v->host_ptr_sync();
@@ -215,6 +217,7 @@ int64_t gsbn_solver_vector_info(gsbn_solver* s, const char* name, int* dtype) {
 int64_t gsbn_solver_vector_read(gsbn_solver* s, const char* name, void* out, int64_t cap) {
   try {
     if (!s || !s->s || !name) throw Error(GSBN_ERR_ARG, "null argument");
+    gsbn::active_stream() = s->s->stream(); /* host copies sync on this solver's stream */
     gsbn::VecBase* v = s->s->db().find(name);
     if (!v) throw Error(GSBN_ERR_NAME, std::string("no vector named ") + name);
     void* p = v->host_ptr_sync();
@@ -229,6 +232,7 @@ int64_t gsbn_solver_vector_read(gsbn_solver* s, const char* name, void* out, int
 int gsbn_solver_vector_write(gsbn_solver* s, const char* name, const void* in, int64_t count) {
   return guard([&] {
     if (!s || !s->s || !name || (!in && count)) throw Error(GSBN_ERR_ARG, "null argument");
+    gsbn::active_stream() = s->s->stream(); /* host copies sync on this solver's stream */
     gsbn::VecBase* v = s->s->db().find(name);
     if (!v) throw Error(GSBN_ERR_NAME, std::string("no vector named ") + name);
     if (count != (int64_t)v->size())

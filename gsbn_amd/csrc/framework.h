@@ -124,6 +124,11 @@ class VecBase {
     static uint64_t e = 0;
     return e;
   }
+  /* the same for host writes only */
+  static uint64_t& host_write_epoch() {
+    static uint64_t e = 0;
+    return e;
+  }
 };
 
 template <typename T>
@@ -185,6 +190,7 @@ class SyncVector : public VecBase {
     status = HOST_NEWER;
     ++host_version;
     ++host_epoch();
+    ++host_write_epoch();
   }
 
   void* host_ptr_sync() override {

@@ -190,6 +190,117 @@ __global__ void pop_dsup0_kernel(gsbn_pop_args a) {
   a.scratch_dsup0[0] = unit_dsup(a, 0, 0, efin, &u, &xi);
 }
 
+/* the seven synapse coefficients of gap D: two 16-byte loads (entries are
+ * 48 bytes, 16-byte aligned) */
+__device__ __forceinline__ gsbn_syn_coef_t load_coef(const float* table, int len, int D) {
+  const float* c = gsbn_coef(table, len, D);
+  const float4 x = *reinterpret_cast<const float4*>(c);
+  const float4 y = *reinterpret_cast<const float4*>(c + 4);
+  gsbn_syn_coef_t k;
+  k.ezi = x.x;
+  k.ezj = x.y;
+  k.ee = x.z;
+  k.ep = x.w;
+  k.cez = y.x;
+  k.cpe = y.y;
+  k.cpz = y.z;
+  return k;
+}
+
+/* ---- synapse state access for the FP32 record and the 16-bit forms ---- */
+struct SynVals {
+  float pij, eij, zi2, zj2, wij;
+  int tij;
+};
+
+template <int PREC>
+struct SynIO {
+  gsbn_syn_rec* syn;
+  gsbn_syn16_rec* syn16;
+  float* wij;
+  float ps, es, pinv, einv;
+
+  template <class A>
+  __device__ __forceinline__ explicit SynIO(const A& a)
+      : syn(a.syn), syn16(a.syn16), wij(a.wij), ps(a.p_scale), es(a.e_scale) {
+    pinv = PREC == GSBN_PREC_F16 ? 1.0f / ps : 1.0f;
+    einv = PREC == GSBN_PREC_F16 ? 1.0f / es : 1.0f;
+  }
+
+  __device__ __forceinline__ void unpack16(const uint4 v, SynVals& s) const {
+    s.pij = gsbn_unpack_trace(PREC, (uint16_t)(v.x & 0xffffu), pinv);
+    s.eij = gsbn_unpack_trace(PREC, (uint16_t)(v.x >> 16), einv);
+    s.zi2 = __uint_as_float(v.y);
+    s.zj2 = __uint_as_float(v.z);
+    s.tij = (int)v.w;
+  }
+  __device__ __forceinline__ uint4 pack16(const SynVals& s) const {
+    const uint32_t p = gsbn_pack_trace(PREC, s.pij, ps);
+    const uint32_t e = gsbn_pack_trace(PREC, s.eij, es);
+    return make_uint4(p | (e << 16), __float_as_uint(s.zi2), __float_as_uint(s.zj2), (uint32_t)s.tij);
+  }
+
+  /* whole state; the weight is read only when need_w (16-bit forms) */
+  __device__ __forceinline__ SynVals load(int64_t i, bool need_w) const {
+    SynVals s;
+    if constexpr (PREC == GSBN_PREC_F32) {
+      const gsbn_syn_rec r = syn[i];
+      s.pij = r.pij; s.eij = r.eij; s.zi2 = r.zi2; s.zj2 = r.zj2; s.wij = r.wij; s.tij = r.tij;
+    } else {
+      unpack16(*reinterpret_cast<const uint4*>(syn16 + i), s);
+      s.wij = need_w ? wij[i] : 0.0f;
+    }
+    return s;
+  }
+  /* whole state; the weight is written only when w_new (16-bit forms) */
+  __device__ __forceinline__ void store(int64_t i, const SynVals& s, bool w_new) const {
+    if constexpr (PREC == GSBN_PREC_F32) {
+      gsbn_syn_rec r;
+      r.pij = s.pij; r.eij = s.eij; r.zi2 = s.zi2; r.zj2 = s.zj2; r.wij = s.wij; r.tij = s.tij;
+      r.pad0 = 0; r.pad1 = 0;
+      syn[i] = r;
+    } else {
+      *reinterpret_cast<uint4*>(syn16 + i) = pack16(s);
+      if (w_new) wij[i] = s.wij;
+    }
+  }
+  /* the traces and tij only (column updates never touch the weight) */
+  __device__ __forceinline__ SynVals load_traces(int64_t i) const {
+    SynVals s;
+    if constexpr (PREC == GSBN_PREC_F32) {
+      const float4 v = *reinterpret_cast<const float4*>(syn + i);
+      s.pij = v.x; s.eij = v.y; s.zi2 = v.z; s.zj2 = v.w; s.wij = 0.0f;
+      s.tij = syn[i].tij;
+    } else {
+      unpack16(*reinterpret_cast<const uint4*>(syn16 + i), s);
+      s.wij = 0.0f;
+    }
+    return s;
+  }
+  __device__ __forceinline__ void store_traces(int64_t i, const SynVals& s) const {
+    if constexpr (PREC == GSBN_PREC_F32) {
+      *reinterpret_cast<float4*>(syn + i) = make_float4(s.pij, s.eij, s.zi2, s.zj2);
+      syn[i].tij = s.tij;
+    } else {
+      *reinterpret_cast<uint4*>(syn16 + i) = pack16(s);
+    }
+  }
+  __device__ __forceinline__ void store_tij(int64_t i, int t) const {
+    if constexpr (PREC == GSBN_PREC_F32)
+      syn[i].tij = t;
+    else
+      syn16[i].tij = t;
+  }
+};
+
+/* per-launch values of a row/column update (taken from the argument struct
+ * in the single form, from the step arguments in the batched form) */
+struct RcStep {
+  int t, exec, P, table_len;
+  float kp;
+  const float* table;
+};
+
 /* What the fused step's row/column phase takes from its population phase
  * (all in LDS): the spike flags, the post-jxx pj and the spike list of the
  * hypercolumn, and the active-row bitmap and list of its one incoming
@@ -204,6 +315,110 @@ struct FusedLds {
   int nrows;
 };
 
+/* Chained step: the previous executed step's row updates of hypercolumn h
+ * (the row pass of rowcol_body with the same arithmetic), done by the
+ * workgroup of this step's population update.  Thread j (< M) updates
+ * synapse j of every active row in ascending row order and adds the row's
+ * contribution to e, the support sum of unit j, so the sum is the one the
+ * population update would form from memory (ascending row order).  kpre:
+ * the first four entries of the row list, read ahead; sjp: unit j's spike
+ * of the previous step; pjv: its pj after that step's update.  Thread 0
+ * writes each batch's row records once every thread has read them.  Called
+ * by every thread of the workgroup (nrows is uniform). */
+template <int PREC>
+__device__ __forceinline__ float chained_rows(const gsbn_rowcol_args& b, const RcStep& q, int h,
+                                              int j, bool live, int nrows, const int* kpre,
+                                              int sjp, float pjv, float e) {
+  const SynIO<PREC> io(b);
+  const int M = b.M, C = b.C;
+  const bool learn = q.kp != 0.0f;
+  const int t = q.t;
+  const int32_t* rl = b.rows_list + (int64_t)h * C;
+  for (int base = 0; base < nrows; base += 4) {
+    int k[4];
+    gsbn_row_rec st[4];
+    SynVals sv[4];
+    float cr[4][GSBN_COEF_STRIDE];
+    gsbn_syn_coef_t cs[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) k[u] = base == 0 ? kpre[u] : (base + u < nrows ? rl[base + u] : 0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (base + u < nrows) {
+        const int64_t r = (int64_t)h * C + k[u];
+        st[u] = b.rowst[r];
+        if (live) sv[u] = io.load(r * M + j, !learn);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (base + u < nrows) {
+        const int Dr = t - st[u].ti;
+        if (Dr > 0) {
+          const float4* c4 =
+              reinterpret_cast<const float4*>(gsbn_coef(q.table, q.table_len, Dr));
+#pragma unroll
+          for (int w = 0; w < GSBN_COEF_STRIDE / 4; ++w) {
+            const float4 x = c4[w];
+            cr[u][4 * w] = x.x;
+            cr[u][4 * w + 1] = x.y;
+            cr[u][4 * w + 2] = x.z;
+            cr[u][4 * w + 3] = x.w;
+          }
+        }
+        if (live) {
+          const int D = t - sv[u].tij;
+          if (D > 0) cs[u] = load_coef(q.table, q.table_len, D);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (base + u >= nrows) break;
+      const int64_t r = (int64_t)h * C + k[u];
+      const int Dr = t - st[u].ti;
+      if (Dr <= 0) {
+        st[u].zi += b.kfti;
+        st[u].ti = t;
+      } else {
+        gsbn_row_catchup(cr[u], &st[u].pi, &st[u].ei, &st[u].zi);
+        st[u].zi = st[u].zi + b.kfti;
+        st[u].ti = t;
+      }
+      if (live) {
+        SynVals& s = sv[u];
+        float cval;
+        bool w_new = false;
+        const int D = t - s.tij;
+        if (D <= 0) {
+          s.zi2 += b.kfti;
+          s.tij = t;
+          cval = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
+        } else {
+          gsbn_syn_catchup_k(cs[u], &s.pij, &s.eij, &s.zi2, &s.zj2);
+          s.zi2 = s.zi2 + b.kfti;
+          s.tij = t;
+          if (learn) {
+            s.wij = gsbn_wij(s.pij, st[u].pi, pjv, b.wgain, b.eps, b.eps2);
+            w_new = true;
+          }
+          cval = s.wij;
+        }
+        if (sjp > 0) s.zj2 += b.kftj;
+        io.store(r * M + j, s, w_new);
+        if (gsbn_fbits(cval) != GSBN_NO_CONTRIB_BITS) e = e + cval;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (base + u < nrows) b.rowst[(int64_t)h * C + k[u]] = st[u];
+    }
+  }
+  return e;
+}
+
 /* One hypercolumn's population update.  FUSED: `f` also receives what the
  * fused step's row/column phase needs (one incoming projection).
  *
@@ -213,9 +428,11 @@ struct FusedLds {
  * contributions are read before their count is known), and every global
  * store of the update is issued after the last load: loads and stores share
  * one in-order counter, so a load issued behind stores would wait for them. */
-template <bool FUSED>
+template <bool FUSED, int CPREC = -1>
 __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
-                                         FusedLds* f = nullptr) {
+                                         FusedLds* f = nullptr,
+                                         const gsbn_rowcol_args* rb = nullptr,
+                                         const RcStep* rq = nullptr, const int has_prev = 0) {
   /* dynamic LDS: three arrays of blockDim entries, then the active-row
    * bitmap (and, FUSED, the active-row list); see pop_lds() */
   extern __shared__ __attribute__((aligned(16))) uint32_t sh_dyn[];
@@ -279,6 +496,20 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     if (j + 2 * bd < p0.W) fut2 = wslot[j + 2 * bd];
     if (j + 3 * bd < p0.W) fut3 = wslot[j + 3 * bd];
   }
+  /* chained step: the previous step's active rows (count, first four
+   * entries) and this unit's spike of that step */
+  const bool chain = CPREC >= 0 && has_prev;
+  int nrp = 0, sjp = 0;
+  int kpre[4] = {0, 0, 0, 0};
+  if (CPREC >= 0 && has_prev) {
+    const int Cp = rb->C;
+    nrp = rb->rows_cnt[h];
+    const int32_t* rl = rb->rows_list + (int64_t)h * Cp;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (u < Cp) kpre[u] = rl[u];
+    if (live) sjp = rb->sj[idx];
+  }
   const float d0 = a.wta_pop_max ? a.scratch_dsup0[0] : 0.0f;
   const bool ext_on = a.ext_off && a.simstep >= 0 && a.simstep < a.ext_nsteps;
   int ext_b = 0, ext_e = 0;
@@ -314,9 +545,12 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   if (ext_on && ext_b + j < ext_e) ext_u0 = a.ext_units[ext_b + j];
 
   /* 1. pending support contributions of the previous step's row updates,
-   * added in ascending row order */
+   * added in ascending row order (chained: computed here from the rows) */
   float efin[GSBN_MAX_PROJ_IN];
-  for (int m = 0; m < nin; ++m) {
+  if constexpr (CPREC >= 0) {
+    if (chain) efin[0] = chained_rows<CPREC>(*rb, *rq, h, j, live, nrp, kpre, sjp, pj0, e0);
+  }
+  for (int m = chain ? 1 : 0; m < nin; ++m) {
     if (!live) {
       efin[m] = 0.0f;
       continue;
@@ -613,121 +847,10 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_arg
   if (j < a.nproj_in) a.projs[j].pend_cnt[h] = 0;
 }
 
-/* the seven synapse coefficients of gap D: two 16-byte loads (entries are
- * 48 bytes, 16-byte aligned) */
-__device__ __forceinline__ gsbn_syn_coef_t load_coef(const float* table, int len, int D) {
-  const float* c = gsbn_coef(table, len, D);
-  const float4 x = *reinterpret_cast<const float4*>(c);
-  const float4 y = *reinterpret_cast<const float4*>(c + 4);
-  gsbn_syn_coef_t k;
-  k.ezi = x.x;
-  k.ezj = x.y;
-  k.ee = x.z;
-  k.ep = x.w;
-  k.cez = y.x;
-  k.cpe = y.y;
-  k.cpz = y.z;
-  return k;
-}
-
-/* ---- synapse state access for the FP32 record and the 16-bit forms ---- */
-struct SynVals {
-  float pij, eij, zi2, zj2, wij;
-  int tij;
-};
-
-template <int PREC>
-struct SynIO {
-  gsbn_syn_rec* syn;
-  gsbn_syn16_rec* syn16;
-  float* wij;
-  float ps, es, pinv, einv;
-
-  template <class A>
-  __device__ __forceinline__ explicit SynIO(const A& a)
-      : syn(a.syn), syn16(a.syn16), wij(a.wij), ps(a.p_scale), es(a.e_scale) {
-    pinv = PREC == GSBN_PREC_F16 ? 1.0f / ps : 1.0f;
-    einv = PREC == GSBN_PREC_F16 ? 1.0f / es : 1.0f;
-  }
-
-  __device__ __forceinline__ void unpack16(const uint4 v, SynVals& s) const {
-    s.pij = gsbn_unpack_trace(PREC, (uint16_t)(v.x & 0xffffu), pinv);
-    s.eij = gsbn_unpack_trace(PREC, (uint16_t)(v.x >> 16), einv);
-    s.zi2 = __uint_as_float(v.y);
-    s.zj2 = __uint_as_float(v.z);
-    s.tij = (int)v.w;
-  }
-  __device__ __forceinline__ uint4 pack16(const SynVals& s) const {
-    const uint32_t p = gsbn_pack_trace(PREC, s.pij, ps);
-    const uint32_t e = gsbn_pack_trace(PREC, s.eij, es);
-    return make_uint4(p | (e << 16), __float_as_uint(s.zi2), __float_as_uint(s.zj2), (uint32_t)s.tij);
-  }
-
-  /* whole state; the weight is read only when need_w (16-bit forms) */
-  __device__ __forceinline__ SynVals load(int64_t i, bool need_w) const {
-    SynVals s;
-    if constexpr (PREC == GSBN_PREC_F32) {
-      const gsbn_syn_rec r = syn[i];
-      s.pij = r.pij; s.eij = r.eij; s.zi2 = r.zi2; s.zj2 = r.zj2; s.wij = r.wij; s.tij = r.tij;
-    } else {
-      unpack16(*reinterpret_cast<const uint4*>(syn16 + i), s);
-      s.wij = need_w ? wij[i] : 0.0f;
-    }
-    return s;
-  }
-  /* whole state; the weight is written only when w_new (16-bit forms) */
-  __device__ __forceinline__ void store(int64_t i, const SynVals& s, bool w_new) const {
-    if constexpr (PREC == GSBN_PREC_F32) {
-      gsbn_syn_rec r;
-      r.pij = s.pij; r.eij = s.eij; r.zi2 = s.zi2; r.zj2 = s.zj2; r.wij = s.wij; r.tij = s.tij;
-      r.pad0 = 0; r.pad1 = 0;
-      syn[i] = r;
-    } else {
-      *reinterpret_cast<uint4*>(syn16 + i) = pack16(s);
-      if (w_new) wij[i] = s.wij;
-    }
-  }
-  /* the traces and tij only (column updates never touch the weight) */
-  __device__ __forceinline__ SynVals load_traces(int64_t i) const {
-    SynVals s;
-    if constexpr (PREC == GSBN_PREC_F32) {
-      const float4 v = *reinterpret_cast<const float4*>(syn + i);
-      s.pij = v.x; s.eij = v.y; s.zi2 = v.z; s.zj2 = v.w; s.wij = 0.0f;
-      s.tij = syn[i].tij;
-    } else {
-      unpack16(*reinterpret_cast<const uint4*>(syn16 + i), s);
-      s.wij = 0.0f;
-    }
-    return s;
-  }
-  __device__ __forceinline__ void store_traces(int64_t i, const SynVals& s) const {
-    if constexpr (PREC == GSBN_PREC_F32) {
-      *reinterpret_cast<float4*>(syn + i) = make_float4(s.pij, s.eij, s.zi2, s.zj2);
-      syn[i].tij = s.tij;
-    } else {
-      *reinterpret_cast<uint4*>(syn16 + i) = pack16(s);
-    }
-  }
-  __device__ __forceinline__ void store_tij(int64_t i, int t) const {
-    if constexpr (PREC == GSBN_PREC_F32)
-      syn[i].tij = t;
-    else
-      syn16[i].tij = t;
-  }
-};
-
-/* per-launch values of a row/column update (taken from the argument struct
- * in the single form, from the step arguments in the batched form) */
-struct RcStep {
-  int t, exec, P, table_len;
-  float kp;
-  const float* table;
-};
-
 /* Part `part` of the row/column update of hypercolumn h.  FUSED: the
  * hypercolumn's active rows, spikes and pj come from the population phase
  * of the same workgroup (fl), not from memory. */
-template <int PREC, bool FUSED = false>
+template <int PREC, bool FUSED = false, bool ROWS = true>
 __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcStep& q,
                                             const int h, const int part,
                                             const FusedLds* fl = nullptr) {
@@ -748,10 +871,10 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
     nrows = fl->nrows;
     nspk = fl->nspk;
   } else {
-    nrows = a.rows_cnt[h];
+    nrows = ROWS ? a.rows_cnt[h] : 0;
     nspk = a.spk_cnt[h];
     const int aa0 = part + P * (tid / M);
-    if (tid / M < rpt && aa0 < C) k_pre = a.rows_list[(int64_t)h * C + aa0];
+    if (ROWS && tid / M < rpt && aa0 < C) k_pre = a.rows_list[(int64_t)h * C + aa0];
     j_pre = a.spk_list[h * M];
   }
   const int mine = nrows > part ? (nrows - part + P - 1) / P : 0;
@@ -929,6 +1052,26 @@ __global__ void __launch_bounds__(RC_THREADS)
   const RcStep q = {st.simstep, st.exec, 1, b.table_lens[st.kp_slot], b.taupdt * st.prn,
                     b.tables[st.kp_slot]};
   rowcol_body<PREC, true>(b, q, h, 0, &f);
+}
+
+/* Chained step, one launch per executed step: workgroup h < H runs the
+ * previous step's row updates of hypercolumn h (when has_prev) and this
+ * step's population update; the H * b.parts further workgroups run the
+ * previous step's column updates.  The two halves touch disjoint state: the
+ * population update reads no synapse record, column updates skip the rows
+ * the row updates own, and the previous step's spike lists and row stamps
+ * (b) are not the ones this step's population update writes (a). */
+template <int PREC>
+__global__ void __launch_bounds__(RC_THREADS) chained_kernel(gsbn_pop_args a, gsbn_rowcol_args b,
+                                                             int has_prev) {
+  const RcStep q = {b.simstep, b.exec, b.parts, b.table_len, b.kp, b.table};
+  if ((int)blockIdx.x < a.H) {
+    pop_body<false, PREC>(a, blockIdx.x, nullptr, &b, &q, has_prev);
+    return;
+  }
+  const int w = blockIdx.x - a.H;
+  const int h = w / b.parts;
+  rowcol_body<PREC, false, false>(b, q, h, w - h * b.parts);
 }
 
 template <int PREC>
@@ -1456,6 +1599,64 @@ int gsbn_step_fused(const gsbn_pop_args* a, const gsbn_rowcol_args* b, void* str
       break;
   }
   return launch_status();
+}
+
+int gsbn_step_chained(const gsbn_pop_args* a, const gsbn_rowcol_args* b, int32_t has_prev,
+                      void* stream) {
+  if (!a || !b || a->H <= 0 || a->M <= 0 || a->M > RC_THREADS || a->nproj_in != 1 ||
+      a->wta_pop_max || a->nproj_out < 0 || (a->nproj_out > 0 && !a->outs))
+    return GSBN_ERR_ARG;
+  if (has_prev && (b->H != a->H || b->M != a->M || b->C != a->proj0.C || b->C < 0 || !b->table ||
+                   b->table_len < 2 || !trace_form_ok(*b) || b->parts < 1))
+    return GSBN_ERR_ARG;
+  /* the previous step's lists and stamps must not be this step's */
+  if (has_prev && (b->spk_list == a->spk_list || b->spk_cnt == a->spk_cnt ||
+                   b->act_stamp == a->proj0.act_stamp || !b->rows_list || !b->rows_cnt ||
+                   !b->rowst || !b->sj))
+    return GSBN_ERR_ARG;
+  const size_t lds = pop_lds(RC_THREADS, a->max_slots, false);
+  if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
+  const int64_t grid = (int64_t)a->H * (1 + (has_prev ? b->parts : 0));
+  if (grid > 0x7fffffff) return GSBN_ERR_UNSUPPORTED;
+  const dim3 g((unsigned)grid), t(RC_THREADS);
+  hipStream_t s = (hipStream_t)stream;
+  const int hp = has_prev ? 1 : 0;
+  /* without a previous step the form only selects the instance */
+  switch (has_prev ? b->prec : GSBN_PREC_F32) {
+    case GSBN_PREC_F32:
+      launch_timed(chained_kernel<GSBN_PREC_F32>, g, t, lds, s, *a, *b, hp);
+      break;
+    case GSBN_PREC_BF16:
+      launch_timed(chained_kernel<GSBN_PREC_BF16>, g, t, lds, s, *a, *b, hp);
+      break;
+    default:
+      launch_timed(chained_kernel<GSBN_PREC_F16>, g, t, lds, s, *a, *b, hp);
+      break;
+  }
+  return launch_status();
+}
+
+int gsbn_chained_resident(int32_t prec, int32_t* out) {
+  if (!out) return GSBN_ERR_ARG;
+  int dev = 0, cus = 0, per = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) {
+    switch (prec) {
+      case GSBN_PREC_F32:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, chained_kernel<GSBN_PREC_F32>, RC_THREADS, 0);
+        break;
+      case GSBN_PREC_BF16:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, chained_kernel<GSBN_PREC_BF16>, RC_THREADS, 0);
+        break;
+      default:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, chained_kernel<GSBN_PREC_F16>, RC_THREADS, 0);
+        break;
+    }
+  }
+  if (e != hipSuccess) return ret(e);
+  *out = per * cus;
+  return 0;
 }
 
 int gsbn_step_fused_batch(const gsbn_pop_args* pops_dev, int32_t npop,

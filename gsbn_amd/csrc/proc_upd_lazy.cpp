@@ -92,6 +92,12 @@ class Pop {
   SyncVector<int32_t>* _counter = nullptr;
   vector<Proj*> _in; /* incoming projections in proj_in_pop order */
   DevBuf<int32_t> spk_list, spk_cnt, ext_off, ext_units;
+  /* second spike-list set of the chained step (steps alternate between the
+   * two), and the set the last executed step wrote */
+  DevBuf<int32_t> spk_list1, spk_cnt1;
+  int spk_side = 0;
+  int32_t* spk_list_at(int side) { return side ? spk_list1.p : spk_list.p; }
+  int32_t* spk_cnt_at(int side) { return side ? spk_cnt1.p : spk_cnt.p; }
   vector<Proj*> _out;       /* projections with this population as source */
   bool push_direct = false; /* this population's kernel does their queue push */
   DevBuf<gsbn_fanout> outs_dev;
@@ -204,6 +210,14 @@ class Proj : public ProjRowOps {
   DevBuf<gsbn_row_rec> rowst;
   DevBuf<float> contrib;
   DevBuf<int32_t> rows_list, rows_cnt, pend_cnt, act_stamp;
+  DevBuf<int32_t> act_stamp1; /* second stamp set of the chained step */
+  int32_t* act_stamp_at(int side) { return side ? act_stamp1.p : act_stamp.p; }
+  /* completes row/column work left pending by a chained step; run before
+   * any read or change of the synapse and row state */
+  std::function<void()> flush_hook;
+  void flush_deferred() {
+    if (flush_hook) flush_hook();
+  }
   DevBuf<uint8_t> sp_flags; /* structural plasticity scratch */
   DevBuf<int32_t> sp_rows;
   /* delay queue: rows of each source unit (CSR) with their delays, and the
@@ -253,6 +267,7 @@ struct SynField : VecView {
   int field;
   SynField(Proj* q, int f) : p(q), field(f) {}
   void pull(VecBase* v) override {
+    p->flush_deferred();
     const size_t n = (size_t)p->nsyn;
     DevBuf<uint32_t> tmp;
     tmp.alloc(n);
@@ -273,6 +288,7 @@ struct RowField : VecView {
   int field;
   RowField(Proj* q, int f) : p(q), field(f) {}
   void pull(VecBase* v) override {
+    p->flush_deferred();
     const size_t n = p->rowst.n;
     DevBuf<uint32_t> tmp;
     tmp.alloc(n);
@@ -324,8 +340,10 @@ void QueueView::pull(VecBase* v) {
     Pop* d = p->_ptr_dest_pop;
     const int M = d->_dim_mcu;
     vector<int32_t> cnt(d->_dim_hcu), lst((size_t)d->_dim_hcu * M);
-    d->spk_cnt.download(cnt.data(), cnt.size());
-    d->spk_list.download(lst.data(), lst.size());
+    DevBuf<int32_t>& dc = d->spk_side ? d->spk_cnt1 : d->spk_cnt;
+    DevBuf<int32_t>& dl = d->spk_side ? d->spk_list1 : d->spk_list;
+    dc.download(cnt.data(), cnt.size());
+    dl.download(lst.data(), lst.size());
     auto& out = raw<int32_t>(v);
     out.clear();
     for (int h = 0; h < d->_dim_hcu; ++h)
@@ -567,6 +585,8 @@ gsbn_pop_args Pop::args(int simstep, int exec, float prn, int lginp_idx, int wma
 
 /* apply pending support contributions before a host read of epsc */
 void Pop::flush() {
+  /* row updates a chained step left pending write the contributions */
+  for (Proj* q : _in) q->flush_deferred();
   if (!flush_pending) return;
   flush_pending = false;
   gsbn_pop_args a;
@@ -1010,6 +1030,7 @@ Proj::~Proj() {
 }
 
 void Proj::prune_flags(int t_min, float wp, float wn, vector<uint8_t>* flags) {
+  flush_deferred();
   const int64_t R = (int64_t)_dim_hcu * _dim_conn;
   prepare(0); /* host writes of ii and the traces reach the device first */
   gsbn_prune_args a;
@@ -1032,6 +1053,7 @@ void Proj::prune_flags(int t_min, float wp, float wn, vector<uint8_t>* flags) {
 }
 
 void Proj::clear_rows(const vector<int32_t>& rows) {
+  flush_deferred();
   if (rows.empty()) return;
   const int64_t R = (int64_t)_dim_hcu * _dim_conn;
   for (int32_t r : rows)
@@ -1057,6 +1079,7 @@ void Proj::clear_rows(const vector<int32_t>& rows) {
 }
 
 void Proj::rows_added() {
+  flush_deferred();
   /* the fan-out lists follow ii/di now */
   prepare(0);
   /* with the population kernel doing the queue push, the spikes of the last
@@ -1107,7 +1130,7 @@ class ProcUpdLazy : public ProcedureBase {
   /* cached launch arguments */
   bool _batch = false;
   bool _args_dirty = true;
-  uint64_t _epoch_seen = ~0ull, _tables_seen = ~0ull;
+  uint64_t _epoch_seen = ~0ull, _tables_seen = ~0ull, _write_epoch_seen = ~0ull;
   vector<gsbn_pop_args> _pop_args;
   vector<gsbn_rowcol_args> _proj_args;
   float _slot_prn[GSBN_KP_SLOTS] = {0, 0, 0, 0};
@@ -1120,6 +1143,15 @@ class ProcUpdLazy : public ProcedureBase {
   int _pop_threads = 64, _max_slots = 0, _any_quirk = 0, _nwg_pop = 0, _nhcu_proj = 0, _parts = 1;
 
   int _max_m = 0, _parts0 = 1;
+  /* Chained step (plan -2): one launch per step runs the previous step's
+   * row and column updates with this step's population update
+   * (gsbn_step_chained); this step's are pending until the next launch or a
+   * flush.  Possible for one population with its one recurrent projection
+   * (direct queue push, M <= 256, no wta_pop_max). */
+  bool _chainable = false, _pending = false;
+  int _chain_fixed = 0, _chain_parts = 1, _chain_warm = 0;
+  gsbn_rowcol_args _pend_ra;
+  void flush_deferred();
   /* every population has M <= 256 and one incoming projection, and the
    * fused step's LDS (active-row bitmap and list) fits */
   bool _fusable = false;
@@ -1150,11 +1182,29 @@ ProcUpdLazy::~ProcUpdLazy() {
   for (auto* p : _list_pop) delete p;
 }
 
+/* the row and column updates a chained step left pending, as a row/column
+ * launch of that step */
+void ProcUpdLazy::flush_deferred() {
+  if (!_pending) return;
+  _pending = false;
+  Proj* q = _list_proj[0];
+  gsbn_rowcol_args r = _pend_ra;
+  const CoefTable& t = q->table(r.kp, (int64_t)r.simstep + 2);
+  r.table = t.dev.p;
+  r.table_len = (int32_t)t.len;
+  r.parts = _parts0;
+  hipStream_t s = active_stream();
+  Timing::get().begin(1, s);
+  op_check(gsbn_proj_rowcol(&r, s), "gsbn_proj_rowcol");
+  Timing::get().end(1, s);
+}
+
 /* the plan of this step; *timed while tuning (the step belongs to run
  * _tune_pos, timed by events _tev[2 _tune_pos] and _tev[2 _tune_pos + 1]) */
 int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
   *timed = false;
   if (_fused_fixed) return -1;
+  if (_chain_fixed > 0) return -2;
   if (_parts_fixed > 0) return _parts_fixed;
   if (_tuned || execs < kTuneStart) return _parts;
   if (_cand.empty()) {
@@ -1171,12 +1221,21 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
     }
     if (_cand.empty()) _cand.push_back(1);
     if (_fusable) _cand.push_back(-1);
+    if (_chainable) _cand.push_back(-2);
     _tev.resize(2 * _cand.size() * kTuneSamples);
     for (auto& e : _tev) hip_check(hipEventCreate(&e), "hipEventCreate");
   }
   (void)s;
+  const int c = _cand[_tune_pos % _cand.size()];
+  /* a chained sample is the second of two chained steps (the first one
+   * leaves the work that a chained launch takes over) */
+  if (c == -2 && !_chain_warm) {
+    _chain_warm = 1;
+    return -2;
+  }
+  _chain_warm = 0;
   *timed = true;
-  return _cand[_tune_pos % _cand.size()];
+  return c;
 }
 
 void ProcUpdLazy::finish_tuning() {
@@ -1216,6 +1275,10 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   par.argi("wta-pop-max", _wta_pop_max);
   par.argi("rowcol-parts", _parts_fixed);
   par.argi("fused-step", _fused_fixed);
+  /* chained-step: 1 always, -1 never, 0 (default) one of the tuned plans */
+  par.argi("chained-step", _chain_fixed);
+  if (_chain_fixed && _fused_fixed > 0 && _chain_fixed > 0)
+    throw Error(-2, "ProcUpdLazy: fused-step and chained-step exclude each other");
   if (_parts_fixed < 0 || _parts_fixed > 1024)
     throw Error(-2, "ProcUpdLazy: rowcol-parts must be in [0, 1024]");
   int hcu_cnt = 0, mcu_cnt = 0;
@@ -1371,16 +1434,52 @@ void ProcUpdLazy::update_gpu() {
     if (_fused_fixed && !_fusable)
       throw Error(-2, "ProcUpdLazy: fused-step needs populations of at most 256 MCUs with one "
                       "incoming projection each, and the slot bitmap and list in 48 KB of LDS");
-    /* starting split until the tuning has run: about 1536 workgroups */
-    _parts0 = std::max(1, std::min(32, 1536 / std::max(_nhcu_proj, 1)));
-    _parts = _fused_fixed ? -1 : _parts0;
-    _glv.puti("rowcol-parts", _fused_fixed ? -1 : _parts_fixed > 0 ? _parts_fixed : _parts);
+    /* starting split until the tuning has run: the largest that the device
+     * holds at once */
+    {
+      int32_t res = 0;
+      const int prec = _list_proj.empty() ? GSBN_PREC_F32 : _list_proj[0]->prec;
+      if (gsbn_rowcol_resident(prec, &res) != 0 || res <= 0) res = 1536;
+      _parts0 = std::max(1, std::min(32, res / std::max(_nhcu_proj, 1)));
+    }
+    _chainable = !_batch && _list_pop.size() == 1 && _list_proj.size() == 1 &&
+                 _list_proj[0]->direct_que && _list_pop[0]->_dim_mcu <= 256 && !_wta_pop_max &&
+                 (3 * 256 + (int64_t)(_max_slots + 31) / 32) * 4 <= 48 * 1024;
+    if (_chain_fixed > 0 && !_chainable)
+      throw Error(-2, "ProcUpdLazy: chained-step needs one population of at most 256 MCUs with "
+                      "one recurrent projection, the population kernel's queue push and no "
+                      "wta-pop-max");
+    if (_chain_fixed < 0) _chainable = false;
+    if (_chainable) {
+      Pop* dp = _list_pop[0];
+      Proj* q = _list_proj[0];
+      const int H = dp->_dim_hcu, M = dp->_dim_mcu;
+      dp->spk_list1.alloc((size_t)H * M);
+      dp->spk_cnt1.alloc((size_t)H);
+      op_check(gsbn_i32_fill(dp->spk_cnt1.p, H, 0, stream), "fill");
+      const int64_t R = (int64_t)q->_dim_hcu * q->_dim_conn;
+      q->act_stamp1.alloc((size_t)R);
+      op_check(gsbn_i32_fill(q->act_stamp1.p, R, -1, stream), "fill");
+      int32_t res = 0;
+      if (gsbn_chained_resident(q->prec, &res) != 0 || res <= 0) res = 768;
+      _chain_parts = std::max(1, std::min(32, (res - H) / std::max(H, 1)));
+      q->flush_hook = [this]() { flush_deferred(); };
+    }
+    _parts = _fused_fixed ? -1 : (_chain_fixed > 0 ? -2 : _parts0);
+    _glv.puti("rowcol-parts", _fused_fixed ? -1
+                              : _chain_fixed > 0 ? -2
+                              : _parts_fixed > 0 ? _parts_fixed : _parts);
     _configured = true;
     _args_dirty = true;
   }
   const int slot = kp_slot(prn);
   for (auto* q : _list_proj) (void)q->table(q->_taupdt * prn, (int64_t)simstep + 2);
   const bool touched = VecBase::host_epoch() != _epoch_seen;
+  /* host writes reach the device below: pending work of a chained step
+   * runs first, on the state it was computed from (reads need no flush:
+   * the views of the synapse and row state flush when they are read) */
+  if (_pending && VecBase::host_write_epoch() != _write_epoch_seen) flush_deferred();
+  _write_epoch_seen = VecBase::host_write_epoch();
   if (touched)
     for (auto* q : _list_proj) q->prepare(simstep);
   for (auto* q : _list_proj)
@@ -1415,8 +1514,10 @@ void ProcUpdLazy::update_gpu() {
   if (sweep) {
     if (timed) _run_len = 0; /* the run restarts after the sweep */
     timed = false;
-    if (plan < 0) plan = _parts0;
+    _chain_warm = 0;
+    if (plan == -1) plan = _parts0;
   }
+  if (plan != -2 && _pending) flush_deferred();
   const size_t ev = 2 * (size_t)_tune_pos;
   if (timed && _run_len == 0) hip_check(hipEventRecord(_tev[ev], stream), "hipEventRecord");
 
@@ -1440,7 +1541,42 @@ void ProcUpdLazy::update_gpu() {
     ra.table_len = ra.table_lens[slot];
     ra.parts = plan < 0 ? 1 : plan;
   }
-  if (plan < 0) {
+  if (plan == -2) {
+    /* chained: this step's lists and stamps go to the set of its exec
+     * parity; the previous step's (the other set) feed its column updates */
+    Pop* dp = _list_pop[0];
+    Proj* q = _list_proj[0];
+    const int side = _exec & 1;
+    pa.spk_list = dp->spk_list_at(side);
+    pa.spk_cnt = dp->spk_cnt_at(side);
+    pa.proj0.act_stamp = q->act_stamp_at(side);
+    if (_pending) {
+      const CoefTable& t = q->table(_pend_ra.kp, (int64_t)_pend_ra.simstep + 2);
+      _pend_ra.table = t.dev.p;
+      _pend_ra.table_len = (int32_t)t.len;
+    } else {
+      _pend_ra = ra; /* not read without a previous step */
+    }
+    Timing::get().begin(6, stream);
+    op_check(gsbn_step_chained(&pa, &_pend_ra, _pending ? 1 : 0, stream), "gsbn_step_chained");
+    Timing::get().end(6, stream);
+    dp->flush_pending = true;
+    dp->spk_side = side;
+    ra.spk_list = pa.spk_list;
+    ra.spk_cnt = pa.spk_cnt;
+    ra.act_stamp = pa.proj0.act_stamp;
+    ra.parts = _chain_parts;
+    _pend_ra = ra;
+    _pending = true;
+    if (sweep) {
+      q->update_all(simstep, old_prn);
+      sweep_synapses += (int64_t)q->_dim_hcu * q->_dim_conn * q->_dim_mcu;
+      sweeps++;
+      q->update_jxx(prn);
+      flush_deferred();
+    }
+  } else if (plan < 0) {
+    for (auto* p : _list_pop) p->spk_side = 0;
     Timing::get().begin(6, stream);
     if (_batch)
       op_check(gsbn_step_fused_batch(_pop_dev.p, (int32_t)_pop_args.size(), _proj_dev.p, _wg_pop.p,
@@ -1452,6 +1588,7 @@ void ProcUpdLazy::update_gpu() {
     Timing::get().end(6, stream);
     for (auto* p : _list_pop) p->flush_pending = true;
   } else {
+    for (auto* p : _list_pop) p->spk_side = 0;
     Timing::get().begin(0, stream);
     if (_batch)
       op_check(gsbn_pop_update_batch(_pop_dev.p, (int32_t)_pop_args.size(), _wg_pop.p,

@@ -266,6 +266,25 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
  * projs_dev[in_proj[p]] of the projection into population p; max_m is the
  * largest M, max_slots the largest slot count. */
 int gsbn_step_fused(const gsbn_pop_args* pop, const gsbn_rowcol_args* proj, void* stream);
+
+/* Chained step (one launch per executed step; one population with its one
+ * recurrent projection, M <= 256, no wta_pop_max): workgroup h < H first
+ * runs the row updates of the previous executed step for hypercolumn h
+ * (`prev`, when has_prev: that step's simstep, exec, kp, table and lists),
+ * adding each updated weight to the support sum of its unit in ascending
+ * row order, then this step's population update (`pop`); H * prev->parts
+ * further workgroups run the previous step's column updates.  This step's
+ * row and column updates are left to the next chained launch or to
+ * gsbn_proj_rowcol with this step's arguments.  `prev`'s spk_list, spk_cnt
+ * and act_stamp must be other arrays than `pop`'s spk_list, spk_cnt and
+ * proj0.act_stamp (two sets, alternated by step), since the column updates
+ * read them while the population update writes its own.  No counterpart in
+ * the reference: its ProcUpdLazy (ProcUpdLazy.cpp:88-160) runs these
+ * updates as separate passes within one step. */
+int gsbn_step_chained(const gsbn_pop_args* pop, const gsbn_rowcol_args* prev, int32_t has_prev,
+                      void* stream);
+/* chained-step workgroups the current device holds at once */
+int gsbn_chained_resident(int32_t prec, int32_t* out);
 int gsbn_step_fused_batch(const gsbn_pop_args* pops_dev, int32_t npop,
                           const gsbn_rowcol_args* projs_dev, const int32_t* wg_pop,
                           const int32_t* wg_hcu, int32_t nwg, const int32_t* in_proj,
