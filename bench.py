@@ -17,8 +17,9 @@ roofline: the row/column kernel, timed with HIP events recorded by each of
 its launches on the solver stream over up to 1000 steps that follow the
 timed region (the timed steps themselves run uninstrumented); algorithmic
 bytes are 44 per synapse update (read pij, eij, zi2, zj2, tij; write them
-and wij).  When the online tuner chose the fused step launch (population
-and row/column update in one kernel), that kernel is the one timed and its
+and wij).  When the online tuner chose a one-launch-per-step plan (the fused
+step, or the chained step that runs the previous step's row/column updates
+with this step's population update), that kernel is the one timed and its
 algorithmic bytes add 40 per unit for the population update.
 
 forms: the same workload with Pij/Eij stored in bf16 and in scaled fp16
@@ -69,6 +70,21 @@ def workload(name: str):
         return configs.config0(), ("lazy BCPNN update, 10 HCU x 10 MCU, dense recurrent "
                                    "projection (9e3 synapses), plain FP32 state")
     raise SystemExit("unknown workload " + name)
+
+
+def apply_plan(spec, plan: str):
+    """--plan: auto leaves the choice to the online tuner."""
+    if plan == "auto":
+        return
+    if plan == "fused":
+        spec.fused_step = 1
+    elif plan == "chained":
+        spec.chained_step = 1
+    elif plan.startswith("split:"):
+        spec.rowcol_parts = int(plan.split(":", 1)[1])
+        spec.chained_step = -1
+    else:
+        raise SystemExit("unknown plan " + plan)
 
 
 def cpu_baseline(args) -> dict:
@@ -157,6 +173,8 @@ def measure(spec, steps, warmup, step_times=0):
     cols = int(w1[1] - w0[1])
     sweep = int(w1[2] - w0[2])
     bpu = BYTES_PER_SYN_UPDATE[spec.trace_precision]
+    # one launch per step (the fused plan, -1, or the chained plan, -2) is
+    # timed as kind 6; split plans time their row/column launches (kind 1)
     fused = int(launches[6]) > int(launches[1])
     kind = 6 if fused else 1
     rc_launch, rc_ms = int(launches[kind]), float(ms[kind])
@@ -169,8 +187,10 @@ def measure(spec, steps, warmup, step_times=0):
             "updates_per_s": (rows + cols + sweep) / el, "steps_per_s": done / el,
             "ms_per_step": 1e3 * el / max(done, 1), "init_s": t_init, "step_ms": step_ms,
             "rc_launches": rc_launch, "rc_avg_ms": avg_ms, "rc_bytes_per_launch": bytes_per_launch,
-            "rc_achieved_gbs": achieved, "rc_parts": abs(parts),
-            "rc_kernel": "step_kernel" if fused else "rowcol_kernel"}
+            "rc_achieved_gbs": achieved, "rc_parts": parts if parts > 0 else None,
+            "rc_plan": {-1: "fused", -2: "chained"}.get(parts, "split"),
+            "rc_kernel": ("chained_kernel" if parts == -2 else "step_kernel") if fused
+            else "rowcol_kernel"}
 
 
 def copy_calibration(mib: int):
@@ -203,7 +223,8 @@ def batched(steps: int, warmup: int) -> dict:
             "synapse_updates_per_sec": m["updates_per_s"],
             "step_ms_p50": m["step_ms"]["p50"], "step_ms_p99": m["step_ms"]["p99"],
             "rowcol_avg_launch_ms": m["rc_avg_ms"], "rowcol_achieved_gbs": m["rc_achieved_gbs"],
-            "rowcol_workgroups_per_hcu": m["rc_parts"], "timed_kernel": m["rc_kernel"]}
+            "rowcol_workgroups_per_hcu": m["rc_parts"], "step_plan": m["rc_plan"],
+            "timed_kernel": m["rc_kernel"]}
 
 
 def frozen_recall(batch: int = 8192, steps: int = 50, clamp_frac: float = 0.4) -> dict:
@@ -277,6 +298,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB five times with the library's copy kernel "
                          "(reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--plan", default="auto",
+                    help="step plan of the single-network runs: auto (tuned online), "
+                         "split:N (N row/column workgroups per HCU), fused or chained")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--step-times", type=int, default=0,
                     help="after the timed region, time this many synchronised single steps")
@@ -291,12 +315,17 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     spec, desc = workload(args.workload)
+    apply_plan(spec, args.plan)
     m = measure(spec, args.steps, args.warmup, args.step_times)
+    # HBM bytes per launch of the timed kernel, from the rocprofv3 counter
+    # runs recorded for this workload (per kernel)
     traffic = None
     tp = os.path.join(ROOT, "profiles", "rowcol_%s_traffic.json" % args.workload)
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            rec = json.load(f).get("kernels", {}).get(m["rc_kernel"])
+        if rec:
+            traffic = rec.get("hbm_bytes_per_launch")
     achieved = m["rc_achieved_gbs"]
     line = {
         "metric": METRIC,
@@ -324,7 +353,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": m["rc_kernel"], "avg_launch_ms": m["rc_avg_ms"],
                      "launches": m["rc_launches"], "bytes_per_launch": m["rc_bytes_per_launch"],
-                     "workgroups_per_hcu": m["rc_parts"]},
+                     "workgroups_per_hcu": m["rc_parts"], "step_plan": m["rc_plan"]},
     }
     if m["step_ms"] is not None:
         line["step_ms"] = m["step_ms"]
@@ -335,13 +364,14 @@ def main():
         for form in ("bf16", "fp16"):
             fspec, _ = workload(args.workload)
             fspec.trace_precision = form
+            apply_plan(fspec, args.plan)
             f = measure(fspec, args.steps, args.warmup)
             forms[form] = {"value": f["updates_per_s"], "steps_per_sec": f["steps_per_s"],
                            "ms_per_step": f["ms_per_step"],
                            "speedup_steps_per_sec": f["steps_per_s"] / m["steps_per_s"],
                            "rowcol_avg_launch_ms": f["rc_avg_ms"],
                            "rowcol_achieved_gbs": f["rc_achieved_gbs"],
-                           "timed_kernel": f["rc_kernel"],
+                           "timed_kernel": f["rc_kernel"], "step_plan": f["rc_plan"],
                            "bytes_per_update": BYTES_PER_SYN_UPDATE[form]}
         line["forms"] = forms
     if args.workload == "100x100" and not args.no_batched:
