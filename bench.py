@@ -213,6 +213,38 @@ def copy_calibration(mib: int):
     del src, dst
 
 
+def hbm_copy(gib: int = 2, reps: int = 10) -> dict:
+    """Device-to-device copy bandwidth with this library's copy kernel
+    (gsbn_device_copy, 16-byte loads and stores): 2 x bytes per copy over
+    the event-timed copies, beside the 8 TB/s the roofline divides by."""
+    import ctypes
+    import torch
+    from gsbn_amd import lib
+    L = lib()
+    L.gsbn_device_copy.restype = ctypes.c_int
+    L.gsbn_device_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.c_void_p]
+    n = gib << 30
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    stream = torch.cuda.current_stream()
+    for _ in range(2):
+        L.gsbn_device_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        if L.gsbn_device_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream) != 0:
+            raise SystemExit("gsbn_device_copy failed")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    ok = bool(torch.equal(src[:1 << 20], dst[:1 << 20]))
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"bytes_per_copy": n, "ms_per_copy": ms, "gbs": 2.0 * n / (ms * 1e-3) / 1e9,
+            "frac_of_8tbs": 2.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "copy_checked": ok}
+
+
 def batched(steps: int, warmup: int) -> dict:
     """The 512-network form: aggregate and per-network step rates."""
     spec, desc = workload("batch512")
@@ -380,6 +412,8 @@ def main():
             line["batched"]["network_steps_per_sec"] / m["steps_per_s"])
     if args.workload == "100x100" and not args.no_recall:
         line["frozen_recall"] = frozen_recall()
+    if args.workload == "100x100" and not args.no_forms:
+        line["hbm_copy"] = hbm_copy()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line))
