@@ -213,36 +213,77 @@ def copy_calibration(mib: int):
     del src, dst
 
 
+def _hip_runtime():
+    """The HIP runtime that libgsbn_amd.so itself loaded (ctypes)."""
+    import ctypes
+    from gsbn_amd import lib
+    lib()
+    path = None
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64.so" in line:
+                path = line.split()[-1]
+                break
+    if path is None:
+        raise SystemExit("libamdhip64 not loaded")
+    return ctypes.CDLL(path)
+
+
 def hbm_copy(gib: int = 2, reps: int = 10) -> dict:
     """Device-to-device copy bandwidth with this library's copy kernel
     (gsbn_device_copy, 16-byte loads and stores): 2 x bytes per copy over
     the event-timed copies, beside the 8 TB/s the roofline divides by."""
     import ctypes
-    import torch
     from gsbn_amd import lib
     L = lib()
+    H = _hip_runtime()
+    vp = ctypes.c_void_p
     L.gsbn_device_copy.restype = ctypes.c_int
-    L.gsbn_device_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
-                                   ctypes.c_void_p]
+    L.gsbn_device_copy.argtypes = [vp, vp, ctypes.c_int64, vp]
+    L.gsbn_i32_fill.restype = ctypes.c_int
+    L.gsbn_i32_fill.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, vp]
+    H.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    H.hipFree.argtypes = [vp]
+    H.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    H.hipEventRecord.argtypes = [vp, vp]
+    H.hipEventSynchronize.argtypes = [vp]
+    H.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    H.hipEventDestroy.argtypes = [vp]
+    H.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    H.hipDeviceSynchronize.argtypes = []
     n = gib << 30
-    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
-    dst = torch.empty(n, dtype=torch.uint8, device="cuda:0")
-    stream = torch.cuda.current_stream()
-    for _ in range(2):
-        L.gsbn_device_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        if L.gsbn_device_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream) != 0:
-            raise SystemExit("gsbn_device_copy failed")
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    ok = bool(torch.equal(src[:1 << 20], dst[:1 << 20]))
-    del src, dst
-    torch.cuda.empty_cache()
-    return {"bytes_per_copy": n, "ms_per_copy": ms, "gbs": 2.0 * n / (ms * 1e-3) / 1e9,
-            "frac_of_8tbs": 2.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "copy_checked": ok}
+    src, dst = vp(), vp()
+    if H.hipMalloc(ctypes.byref(src), n) or H.hipMalloc(ctypes.byref(dst), n):
+        raise SystemExit("hipMalloc failed")
+    try:
+        if L.gsbn_i32_fill(src, n // 4, 0x5a5a1234, None) != 0:
+            raise SystemExit("gsbn_i32_fill failed")
+        for _ in range(2):
+            L.gsbn_device_copy(dst, src, n, None)
+        e0, e1 = vp(), vp()
+        H.hipEventCreate(ctypes.byref(e0))
+        H.hipEventCreate(ctypes.byref(e1))
+        H.hipEventRecord(e0, None)
+        for _ in range(reps):
+            if L.gsbn_device_copy(dst, src, n, None) != 0:
+                raise SystemExit("gsbn_device_copy failed")
+        H.hipEventRecord(e1, None)
+        H.hipEventSynchronize(e1)
+        ms = ctypes.c_float(0.0)
+        H.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        H.hipEventDestroy(e0)
+        H.hipEventDestroy(e1)
+        check = np.zeros(1 << 18, np.int32)
+        H.hipMemcpy(check.ctypes.data, dst, check.nbytes, 2)  # device to host
+        ok = bool((check == 0x5a5a1234).all())
+    finally:
+        H.hipDeviceSynchronize()
+        H.hipFree(src)
+        H.hipFree(dst)
+    t = ms.value / reps
+    gbs = 2.0 * n / (t * 1e-3) / 1e9
+    return {"bytes_per_copy": n, "ms_per_copy": t, "gbs": gbs, "frac_of_8tbs": gbs / HBM_PEAK_GBS,
+            "copy_checked": ok}
 
 
 def batched(steps: int, warmup: int) -> dict:
