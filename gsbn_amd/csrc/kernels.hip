@@ -1066,6 +1066,9 @@ __global__ void __launch_bounds__(RC_THREADS) chained_kernel(gsbn_pop_args a, gs
                                                              int has_prev) {
   const RcStep q = {b.simstep, b.exec, b.parts, b.table_len, b.kp, b.table};
   if ((int)blockIdx.x < a.H) {
+    /* the population half is the longer path: its waves take issue
+     * priority over the column workgroups that share their CUs */
+    __builtin_amdgcn_s_setprio(2);
     pop_body<false, PREC>(a, blockIdx.x, nullptr, &b, &q, has_prev);
     return;
   }
