@@ -381,8 +381,8 @@ def main():
     if args.cpu_only:
         return cpu_only(args)
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU form is a design only "
-              "(see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        print("bench.py: runs on one GPU only (sharding across GPUs by destination "
+              "hypercolumn is a design, not built). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:
