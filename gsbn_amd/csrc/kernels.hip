@@ -442,6 +442,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   uint32_t* const sh_bitmap = sh_dyn + 3 * blockDim.x;
   __shared__ int sh_scan[POP_MAX_THREADS / 64 + 1];
   __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
+  __shared__ float sh_wmax[POP_MAX_THREADS / 64]; /* per-wave WTA maxima */
   __shared__ float sh_vsum;
   __shared__ int sh_sj[FUSED ? RC_THREADS : 1];
   __shared__ float sh_pj[FUSED ? RC_THREADS : 1];
@@ -586,9 +587,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     float sup = a.lgbias + a.igain * lg + xi;
     sup += (a.wgain * wmv) * wsup;
     dsup = dsup_old + (sup - dsup_old) * a.taumdt;
-    sh_f[j] = dsup;
   }
-  __syncthreads();
 
   /* 3. WTA maximum (update_sup_kernel_2_cpu, Pop.cpp:270-279): the
    * reference's sequential `if (d > max) max = d` keeps the first of equal
@@ -602,12 +601,12 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
       const float o = __shfl_down(mv, off, 64);
       if ((j & (2 * off - 1)) == 0 && o > mv) mv = o;
     }
-    if ((j & 63) == 0) sh_f[j >> 6] = mv;
+    if ((j & 63) == 0) sh_wmax[j >> 6] = mv;
   }
   __syncthreads();
-  float maxdsup = sh_f[0];
+  float maxdsup = sh_wmax[0];
   for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
-    const float o = sh_f[w];
+    const float o = sh_wmax[w];
     if (o > maxdsup) maxdsup = o;
   }
   if (a.wta_pop_max) {
@@ -615,7 +614,6 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     if (maxdsup > mx) mx = maxdsup;
     maxdsup = mx;
   }
-  __syncthreads();
 
   /* soft-max (Pop.cpp:280-302); the normaliser is summed in ascending unit
    * order by one thread, four LDS words per read */
@@ -794,7 +792,9 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
                   (unsigned long long)spk_tot * (unsigned long long)C);
       }
     }
-    __syncthreads();
+    /* the next projection reuses the bitmap; after the last one the kernel
+     * ends, or the fused caller synchronises before reading the lists */
+    if (m + 1 < a.nproj_in) __syncthreads();
   }
 }
 
