@@ -188,7 +188,8 @@ def measure(spec, steps, warmup, step_times=0):
             "ms_per_step": 1e3 * el / max(done, 1), "init_s": t_init, "step_ms": step_ms,
             "rc_launches": rc_launch, "rc_avg_ms": avg_ms, "rc_bytes_per_launch": bytes_per_launch,
             "rc_achieved_gbs": achieved, "rc_parts": parts if parts > 0 else None,
-            "rc_plan": {-1: "fused", -2: "chained"}.get(parts, "split"),
+            "rc_plan": ({-1: "fused", -2: "chained"}.get(parts, "split") if parts > -10
+                        else "graded-%d" % (-10 - parts)),
             "rc_kernel": ("chained_kernel" if parts == -2 else "step_kernel") if fused
             else "rowcol_kernel"}
 

@@ -135,6 +135,9 @@ class NetSpec:
     # row/column updates with this step's population update; -1 never;
     # 0: one of the plans chosen online
     chained_step: int = 0
+    # rowcol_graded g (1-3): batches split their row/column work per size
+    # bucket of hypercolumns (plan -10-g); 0: chosen online
+    rowcol_graded: int = 0
 
     @property
     def total_hcu(self) -> int:
@@ -295,6 +298,8 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('  argi { key: "rowcol-parts" val: %d }' % spec.rowcol_parts)
     if spec.chained_step:
         L.append('  argi { key: "chained-step" val: %d }' % spec.chained_step)
+    if spec.rowcol_graded:
+        L.append('  argi { key: "rowcol-graded" val: %d }' % spec.rowcol_graded)
     L.append("}")
     if spec.conn_arrays is None:
         L.append("proc_param {")

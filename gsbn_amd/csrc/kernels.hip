@@ -1022,6 +1022,22 @@ __global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC))
   rowcol_body<PREC>(a, q, wg_hcu[w], blockIdx.x - w * parts);
 }
 
+/* The same with a work split per hypercolumn: workgroup w runs part
+ * wg_part[w] & 0xffff of wg_part[w] >> 16 of hypercolumn wg_hcu[w] of
+ * projection wg_proj[w] (larger hypercolumns split into more parts). */
+template <int PREC>
+__global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC))
+    rowcol_kernel_graded(const gsbn_rowcol_args* __restrict__ arr,
+                         const int32_t* __restrict__ wg_proj, const int32_t* __restrict__ wg_hcu,
+                         const int32_t* __restrict__ wg_part, gsbn_step_args st) {
+  const int w = blockIdx.x;
+  const gsbn_rowcol_args& a = arr[wg_proj[w]];
+  const int pp = wg_part[w];
+  const RcStep q = {st.simstep, st.exec, pp >> 16, a.table_lens[st.kp_slot], a.taupdt * st.prn,
+                    a.tables[st.kp_slot]};
+  rowcol_body<PREC>(a, q, wg_hcu[w], pp & 0xffff);
+}
+
 /* One launch per step: workgroup h updates hypercolumn h's population and
  * then, from LDS, its rows and columns.  Everything the row/column update
  * reads in a step comes from the same workgroup and the queue pushes go to
@@ -1573,6 +1589,31 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
 }
 
 
+
+int gsbn_proj_rowcol_graded(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
+                            const int32_t* wg_hcu, const int32_t* wg_part, int32_t nwg,
+                            const gsbn_step_args* step, void* stream) {
+  if (!projs_dev || !wg_proj || !wg_hcu || !wg_part || !step || nwg <= 0 || step->kp_slot < 0 ||
+      step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 || step->prec > GSBN_PREC_F16)
+    return GSBN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 g((unsigned)nwg), b(RC_THREADS);
+  switch (step->prec) {
+    case GSBN_PREC_F32:
+      launch_timed(rowcol_kernel_graded<GSBN_PREC_F32>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+                   wg_part, *step);
+      break;
+    case GSBN_PREC_BF16:
+      launch_timed(rowcol_kernel_graded<GSBN_PREC_BF16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+                   wg_part, *step);
+      break;
+    default:
+      launch_timed(rowcol_kernel_graded<GSBN_PREC_F16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
+                   wg_part, *step);
+      break;
+  }
+  return launch_status();
+}
 
 int gsbn_step_fused(const gsbn_pop_args* a, const gsbn_rowcol_args* b, void* stream) {
   if (!a || !b || a->H <= 0 || a->M <= 0 || a->M > RC_THREADS || a->nproj_in != 1 || !a->projs ||

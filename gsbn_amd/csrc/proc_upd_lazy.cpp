@@ -1148,6 +1148,18 @@ class ProcUpdLazy : public ProcedureBase {
    * (gsbn_step_chained); this step's are pending until the next launch or a
    * flush.  Possible for one population with its one recurrent projection
    * (direct queue push, M <= 256, no wta_pop_max). */
+  /* Graded splits of a batch (plans -11, -12, -13): the row/column
+   * hypercolumns fall into four size buckets by the quartiles of their work
+   * (slots x MCUs); plan -10-g splits bucket b (0 smallest) into
+   * 2^max(0, b - 3 + g) parts, so g = 3 gives 1, 2, 4, 8 parts.  One map per
+   * plan, built on first use. */
+  int _graded_fixed = 0;
+  vector<int64_t> _hcu_work; /* per row/column HCU, in launch order */
+  vector<int32_t> _proj_map_host, _hcu_map_host;
+  int64_t _bucket_hi[3] = {0, 0, 0};
+  DevBuf<int32_t> _gmap_proj[4], _gmap_hcu[4], _gmap_part[4];
+  int _gmap_n[4] = {0, 0, 0, 0};
+  void build_graded_map(int g);
   bool _chainable = false, _pending = false;
   int _chain_fixed = 0, _chain_parts = 1, _chain_warm = 0;
   gsbn_rowcol_args _pend_ra;
@@ -1205,6 +1217,7 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
   *timed = false;
   if (_fused_fixed) return -1;
   if (_chain_fixed > 0) return -2;
+  if (_graded_fixed > 0) return -10 - _graded_fixed;
   if (_parts_fixed > 0) return _parts_fixed;
   if (_tuned || execs < kTuneStart) return _parts;
   if (_cand.empty()) {
@@ -1222,6 +1235,8 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
     if (_cand.empty()) _cand.push_back(1);
     if (_fusable) _cand.push_back(-1);
     if (_chainable) _cand.push_back(-2);
+    if (_batch && !_hcu_work.empty())
+      for (int g = 1; g <= 3; ++g) _cand.push_back(-10 - g);
     _tev.resize(2 * _cand.size() * kTuneSamples);
     for (auto& e : _tev) hip_check(hipEventCreate(&e), "hipEventCreate");
   }
@@ -1277,6 +1292,10 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   par.argi("fused-step", _fused_fixed);
   /* chained-step: 1 always, -1 never, 0 (default) one of the tuned plans */
   par.argi("chained-step", _chain_fixed);
+  /* rowcol-graded g (1-3): always the graded split g of a batch */
+  par.argi("rowcol-graded", _graded_fixed);
+  if (_graded_fixed < 0 || _graded_fixed > 3)
+    throw Error(-2, "ProcUpdLazy: rowcol-graded must be in [0, 3]");
   if (_chain_fixed && _fused_fixed > 0 && _chain_fixed > 0)
     throw Error(-2, "ProcUpdLazy: fused-step and chained-step exclude each other");
   if (_parts_fixed < 0 || _parts_fixed > 1024)
@@ -1359,13 +1378,22 @@ void ProcUpdLazy::build_batch_maps() {
     const Proj* b = _list_proj[y];
     return (int64_t)a->_dim_conn * a->_dim_mcu > (int64_t)b->_dim_conn * b->_dim_mcu;
   });
+  _hcu_work.clear();
   for (int i : order)
     for (int h = 0; h < _list_proj[i]->_dim_hcu; ++h) {
       qp.push_back((int32_t)i);
       qh.push_back(h);
+      _hcu_work.push_back((int64_t)_list_proj[i]->_dim_conn * _list_proj[i]->_dim_mcu);
     }
+  if (!_hcu_work.empty()) {
+    vector<int64_t> w = _hcu_work;
+    std::sort(w.begin(), w.end());
+    for (int b = 0; b < 3; ++b) _bucket_hi[b] = w[(w.size() * (size_t)(b + 1)) / 4];
+  }
   _nwg_pop = (int)wp.size();
   _nhcu_proj = (int)qp.size();
+  _proj_map_host = qp;
+  _hcu_map_host = qh;
   /* the projection into each population (fused step: exactly one) */
   vector<int32_t> in;
   for (Pop* p : _list_pop)
@@ -1379,6 +1407,24 @@ void ProcUpdLazy::build_batch_maps() {
     _wg_proj.upload(qp.data(), qp.size());
     _wg_proj_hcu.upload(qh.data(), qh.size());
   }
+}
+
+void ProcUpdLazy::build_graded_map(int g) {
+  vector<int32_t> mp, mh, mq;
+  for (size_t i = 0; i < _hcu_work.size(); ++i) {
+    int b = 0;
+    while (b < 3 && _hcu_work[i] > _bucket_hi[b]) ++b;
+    const int parts = 1 << std::max(0, b - 3 + g);
+    for (int part = 0; part < parts; ++part) {
+      mp.push_back(_proj_map_host[i]);
+      mh.push_back(_hcu_map_host[i]);
+      mq.push_back(part | (parts << 16));
+    }
+  }
+  _gmap_proj[g].upload(mp.data(), mp.size());
+  _gmap_hcu[g].upload(mh.data(), mh.size());
+  _gmap_part[g].upload(mq.data(), mq.size());
+  _gmap_n[g] = (int)mp.size();
 }
 
 void ProcUpdLazy::refresh_args(int simstep) {
@@ -1465,9 +1511,12 @@ void ProcUpdLazy::update_gpu() {
       _chain_parts = std::max(1, std::min(32, (res - H) / std::max(H, 1)));
       q->flush_hook = [this]() { flush_deferred(); };
     }
+    if (_graded_fixed && !_batch)
+      throw Error(-2, "ProcUpdLazy: rowcol-graded needs a batch of populations or projections");
     _parts = _fused_fixed ? -1 : (_chain_fixed > 0 ? -2 : _parts0);
     _glv.puti("rowcol-parts", _fused_fixed ? -1
                               : _chain_fixed > 0 ? -2
+                              : _graded_fixed > 0 ? -10 - _graded_fixed
                               : _parts_fixed > 0 ? _parts_fixed : _parts);
     _configured = true;
     _args_dirty = true;
@@ -1575,7 +1624,7 @@ void ProcUpdLazy::update_gpu() {
       q->update_jxx(prn);
       flush_deferred();
     }
-  } else if (plan < 0) {
+  } else if (plan == -1) {
     for (auto* p : _list_pop) p->spk_side = 0;
     Timing::get().begin(6, stream);
     if (_batch)
@@ -1609,7 +1658,13 @@ void ProcUpdLazy::update_gpu() {
     }
     if (!_list_proj.empty()) {
       Timing::get().begin(1, stream);
-      if (_batch)
+      if (_batch && plan <= -10) {
+        const int g = -10 - plan;
+        if (!_gmap_n[g]) build_graded_map(g);
+        op_check(gsbn_proj_rowcol_graded(_proj_dev.p, _gmap_proj[g].p, _gmap_hcu[g].p,
+                                         _gmap_part[g].p, _gmap_n[g], &st, stream),
+                 "gsbn_proj_rowcol_graded");
+      } else if (_batch)
         op_check(gsbn_proj_rowcol_batch(_proj_dev.p, _wg_proj.p, _wg_proj_hcu.p, _nhcu_proj, plan,
                                         &st, stream),
                  "gsbn_proj_rowcol_batch");

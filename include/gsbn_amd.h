@@ -253,6 +253,17 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
                            const int32_t* wg_hcu, int32_t nhcu, int32_t parts,
                            const gsbn_step_args* step, void* stream);
 
+/* Batched row/column update with a work split per hypercolumn: workgroup w
+ * runs part (wg_part[w] & 0xffff) of (wg_part[w] >> 16) parts of hypercolumn
+ * wg_hcu[w] of projection wg_proj[w]; every part of a hypercolumn must be in
+ * the map once.  Used to split large hypercolumns of a batch more finely
+ * than small ones (ProcUpdLazy's split per size bucket).  Replaces, like
+ * gsbn_proj_rowcol_batch, the row and column updates of the reference
+ * (Proj.cu:488-583) for every projection of a batch. */
+int gsbn_proj_rowcol_graded(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
+                            const int32_t* wg_hcu, const int32_t* wg_part, int32_t nwg,
+                            const gsbn_step_args* step, void* stream);
+
 /* One launch per executed step without a sweep (ProcUpdLazy.cpp:112-123
  * and 153-165 in one kernel), for populations with one incoming projection
  * and M <= 256: workgroup h runs the population update of hypercolumn h and

@@ -120,6 +120,20 @@ def test_batched_networks_bitwise(tmp_path):
     gpu.close()
 
 
+@pytest.mark.parametrize("graded", [1, 3])
+def test_batched_networks_graded_split(tmp_path, graded):
+    # the batch's row/column work split per size bucket of hypercolumns
+    # (the largest quarter into 2 resp. 8 parts, the smallest into one)
+    spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=(3, 17), n_pat=4, train=40,
+                                quiet=5, recall=30, reset=20)
+    spec.rowcol_graded = graded
+    gpu, ora = _lockstep_and_compare(spec, tmp_path)
+    c, t, flags = gpu.check()
+    assert (c, t) == (ora.check.correct, ora.check.total)
+    assert gpu.glob("rowcol-parts") == -10 - graded
+    gpu.close()
+
+
 def test_batched_networks_mixed_block_sizes(tmp_path):
     # populations of 40-140 MCUs in one batched population launch of
     # 192-thread blocks (most threads idle for the small ones), the
