@@ -1235,8 +1235,12 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
     if (_cand.empty()) _cand.push_back(1);
     if (_fusable) _cand.push_back(-1);
     if (_chainable) _cand.push_back(-2);
+    /* graded splits; their maps are built here, before any sample is timed */
     if (_batch && !_hcu_work.empty())
-      for (int g = 1; g <= 3; ++g) _cand.push_back(-10 - g);
+      for (int g = 1; g <= 3; ++g) {
+        _cand.push_back(-10 - g);
+        if (!_gmap_n[g]) build_graded_map(g);
+      }
     _tev.resize(2 * _cand.size() * kTuneSamples);
     for (auto& e : _tev) hip_check(hipEventCreate(&e), "hipEventCreate");
   }
