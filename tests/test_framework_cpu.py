@@ -1,6 +1,7 @@
 """CPU checks of the boundary: the library loads and exports every entry
 point include/gsbn_amd.h declares; the prototxt and StimRawData readers
-accept the reference's own files; the workload writer round-trips."""
+accept the reference's own files; the workload writer round-trips, with the
+ProcUpdLazy step-plan keys; bench.py's --plan maps onto them."""
 
 import ctypes
 import os
@@ -109,3 +110,43 @@ def test_solver_without_device_fails_loudly(tmp_path):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
                        cwd=os.path.dirname(HERE), timeout=120)
     assert r.returncode != 0 and "GsbnError" in r.stderr
+
+
+@pytest.mark.parametrize("field,key,value", [("chained_step", "chained-step", 1),
+                                             ("chained_step", "chained-step", -1),
+                                             ("rowcol_graded", "rowcol-graded", 2),
+                                             ("fused_step", "fused-step", 1),
+                                             ("rowcol_parts", "rowcol-parts", 15)])
+def test_step_plan_keys_written(tmp_path, field, key, value):
+    # the ProcUpdLazy plan options reach the prototxt, and the file still parses
+    spec = configs.config0()
+    setattr(spec, field, value)
+    path = configs.write(spec, str(tmp_path))
+    text = open(path).read()
+    assert 'argi { key: "%s" val: %d }' % (key, value) in text
+    L = gsbn_amd.lib()
+    L.gsbn_config_check.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    out = np.zeros(4, np.int32)
+    assert L.gsbn_config_check(path.encode(), out.ctypes.data) == 0, L.gsbn_last_error()
+
+
+def test_bench_plan_option():
+    # bench.py --plan maps onto the same spec fields
+    import importlib.util
+    spec_b = importlib.util.spec_from_file_location("bench_mod", os.path.join(os.path.dirname(HERE),
+                                                                              "bench.py"))
+    bench = importlib.util.module_from_spec(spec_b)
+    spec_b.loader.exec_module(bench)
+    s = configs.config0()
+    bench.apply_plan(s, "auto")
+    assert (s.fused_step, s.chained_step, s.rowcol_parts) == (0, 0, 0)
+    bench.apply_plan(s, "chained")
+    assert s.chained_step == 1
+    s = configs.config0()
+    bench.apply_plan(s, "split:15")
+    assert s.rowcol_parts == 15 and s.chained_step == -1
+    s = configs.config0()
+    bench.apply_plan(s, "fused")
+    assert s.fused_step == 1
+    with pytest.raises(SystemExit):
+        bench.apply_plan(configs.config0(), "nonsense")
