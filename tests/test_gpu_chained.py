@@ -71,6 +71,10 @@ def test_chained_config1_size(tmp_path):
     _check(configs.config1(), tmp_path, max_steps=120)
 
 
+def test_chained_config1_state_every_100_steps(tmp_path):
+    _check(configs.config1(), tmp_path, max_steps=300, check_every=100)
+
+
 def test_chained_sparse_delay_map(tmp_path):
     text = open(os.path.join(os.path.dirname(__file__), "golden", "init_conn_0.1.map")).read()
     spec = configs.recall_net(400, 10, 4, 60, 10, 60, 0.5, reset=20, name="sparse")

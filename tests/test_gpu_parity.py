@@ -109,6 +109,18 @@ def test_config1_size_lockstep(tmp_path):
     gpu.close()
 
 
+def test_config1_tuned_plans_state_every_100_steps(tmp_path):
+    # 100 x 100 for 300 steps while the online tuner alternates its plans
+    # step by step from step 20 (splits of 1-32 parts, the chained launch):
+    # spikes every step, every Database vector every 100 steps
+    spec = configs.config1()
+    gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path, max_steps=300, check_every=100)
+    assert div is None, "diverged at %s" % (div,)
+    bad = helpers.compare_state(gpu, ora, spec)
+    assert not bad, bad
+    gpu.close()
+
+
 def test_batched_networks_bitwise(tmp_path):
     # independent networks of different sizes and connection densities in one
     # batched population launch and one batched row/column launch
