@@ -146,6 +146,18 @@ def test_batched_networks_graded_split(tmp_path, graded):
     gpu.close()
 
 
+def test_batched_networks_tuned_plans(tmp_path):
+    # 24 networks while the tuner alternates the batch plans step by step
+    # (uniform splits, the fused launch, the graded splits) from step 20
+    spec = configs.batched_nets(24, hcu_range=(3, 12), mcu_range=(3, 33), n_pat=4, train=40,
+                                quiet=5, recall=30, reset=20)
+    gpu, ora = _lockstep_and_compare(spec, tmp_path)
+    c, t, flags = gpu.check()
+    assert (c, t) == (ora.check.correct, ora.check.total)
+    assert gpu.glob("step-plan-ms-11", "f") > 0.0  # a graded split was timed
+    gpu.close()
+
+
 def test_batched_networks_mixed_block_sizes(tmp_path):
     # populations of 40-140 MCUs in one batched population launch of
     # 192-thread blocks (most threads idle for the small ones), the
