@@ -1,6 +1,6 @@
 """Structural plasticity on the GPU (-m gpu): ProcStructuralPlasticity
 (pruning on the device, growth on the host) in lockstep with the oracle's
-restatement.  Every step's spikes and, every 20 steps, every Database vector
+restatement, on the 10x10 network and on three batched networks.  Every step's spikes and, every 20 steps, every Database vector
 must be bit-identical, and at the end the fan-out counters.  Cases: the
 10x10 network with pruning every 60 and growth every 20 steps (direct and
 copy-and-queue spike exchange, bf16 form), random initial connectivity grown
@@ -67,6 +67,21 @@ def test_structural_init_conn_and_geometric_delays(tmp_path):
     gpu, ora = _lockstep(spec, tmp_path, 400)
     di = gpu.vector("di_0")
     assert len(set(di[gpu.vector("ii_0") >= 0].tolist())) > 1
+    gpu.close()
+
+
+def test_structural_batched_networks(tmp_path):
+    # three networks of one batch, each pruned and grown on its own
+    spec = configs.batched_nets(3, hcu_range=(3, 6), mcu_range=(3, 9), n_pat=4, train=40,
+                                quiet=5, recall=30, reset=20)
+    for m in spec.modes:
+        m.plasticity = 1
+    spec.structural = configs.StructuralSpec(period=20, pruning_period=60, t_th=30, seed=7)
+    assert spec.conn_arrays
+    ii0 = [np.asarray(ii).copy() for ii, di in spec.conn_arrays]
+    gpu, ora = _lockstep(spec, tmp_path, 300)
+    # rows were removed or grown in at least one network
+    assert any(not np.array_equal(gpu.vector("ii_%d" % i), ii0[i]) for i in range(len(ii0)))
     gpu.close()
 
 
