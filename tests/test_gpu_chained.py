@@ -162,11 +162,15 @@ def test_online_choice_includes_chained(tmp_path):
     gpu.close()
 
 
-@pytest.mark.parametrize("what", ["wta", "batched"])
+@pytest.mark.parametrize("what", ["wta", "batched", "queue"])
 def test_chained_refused(tmp_path, what):
     if what == "wta":
         spec = configs.config0()
         spec.wta_pop_max = 1
+    elif what == "queue":
+        # the population kernel does not push its own spikes
+        spec = configs.config0()
+        spec.exchange_direct = False
     else:
         spec = configs.batched_nets(3, hcu_range=(3, 5), mcu_range=(3, 6), n_pat=2, train=20,
                                     quiet=5, recall=10, reset=10)
