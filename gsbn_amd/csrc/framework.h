@@ -251,11 +251,12 @@ class SyncVector : public VecBase {
 
  private:
   /* host -> device copy in stream order.  Vectors of up to kStageMax bytes
-   * (stimulus rows, masks, counters) go through a pinned staging buffer of
-   * their own, so the call returns without waiting for the device work
-   * queued before it; the buffer is refilled only after its previous copy
-   * has run (an event).  Larger vectors wait for their copy. */
-  static constexpr size_t kStageMax = (size_t)4 << 20;
+   * (stimulus rows of every network of a batch, masks, counters) go through
+   * a pinned staging buffer of their own, so the call returns without
+   * waiting for the device work queued before it; the buffer is refilled
+   * only after its previous copy has run (an event).  Larger vectors wait
+   * for their copy. */
+  static constexpr size_t kStageMax = (size_t)64 << 20;
   void upload() {
     const size_t bytes = n_ * sizeof(T);
     if (bytes <= kStageMax) {

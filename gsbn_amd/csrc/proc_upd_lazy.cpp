@@ -1139,6 +1139,7 @@ class ProcUpdLazy : public ProcedureBase {
   /* batched launch state */
   DevBuf<gsbn_pop_args> _pop_dev;
   DevBuf<gsbn_rowcol_args> _proj_dev;
+  vector<uint8_t> _pop_sent, _proj_sent; /* what the two arrays hold on the device */
   DevBuf<int32_t> _wg_pop, _wg_pop_hcu, _wg_proj, _wg_proj_hcu, _in_proj;
   int _pop_threads = 64, _max_slots = 0, _any_quirk = 0, _nwg_pop = 0, _nhcu_proj = 0, _parts = 1;
 
@@ -1436,9 +1437,22 @@ void ProcUpdLazy::refresh_args(int simstep) {
   for (auto* p : _list_pop) _pop_args.push_back(p->args(0, 0, 0.0f, 0, 0, true, _seed, _wta_pop_max));
   _proj_args.clear();
   for (auto* q : _list_proj) _proj_args.push_back(q->static_args(_slot_prn, _nslots, (int64_t)simstep + 2));
+  /* a host write (a new stimulus row, say) leaves these unchanged: the
+   * upload, which waits for the queued steps, is skipped then */
   if (_batch) {
-    _pop_dev.upload(_pop_args.data(), _pop_args.size());
-    if (!_proj_args.empty()) _proj_dev.upload(_proj_args.data(), _proj_args.size());
+    const size_t pb = _pop_args.size() * sizeof(gsbn_pop_args);
+    if (_pop_sent.size() != pb || memcmp(_pop_sent.data(), _pop_args.data(), pb)) {
+      _pop_dev.upload(_pop_args.data(), _pop_args.size());
+      _pop_sent.assign(reinterpret_cast<const uint8_t*>(_pop_args.data()),
+                       reinterpret_cast<const uint8_t*>(_pop_args.data()) + pb);
+    }
+    const size_t qb = _proj_args.size() * sizeof(gsbn_rowcol_args);
+    if (!_proj_args.empty() &&
+        (_proj_sent.size() != qb || memcmp(_proj_sent.data(), _proj_args.data(), qb))) {
+      _proj_dev.upload(_proj_args.data(), _proj_args.size());
+      _proj_sent.assign(reinterpret_cast<const uint8_t*>(_proj_args.data()),
+                        reinterpret_cast<const uint8_t*>(_proj_args.data()) + qb);
+    }
   }
   _args_dirty = false;
 }
