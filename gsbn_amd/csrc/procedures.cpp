@@ -54,7 +54,6 @@ class ProcExtGen : public ProcedureBase {
   vector<int> _mcu_in_hcu;
   float _eps = 0;
   int _cursor = 0;
-  int _old_lgidx_id = -1;
 };
 REGISTERIMPL(ProcExtGen);
 
@@ -120,9 +119,25 @@ void ProcExtGen::init_new(SolverParam solver_param, Database& db) {
     for (uint32_t k = 0; k < pp.pop_num; ++k)
       for (uint32_t j = 0; j < pp.hcu_num; ++j) _mcu_in_hcu.push_back(pp.mcu_num);
   }
-  _lginp->resize(mcu_num);
+  /* every stimulus row's input (ProcExtGen.cpp:141-150: log(1 + eps) for
+   * the MCU the row names in each HCU, log(eps) elsewhere), built once: a
+   * mode selects its row with lginp-idx, as .wmask's rows are selected by
+   * wmask-idx, so a mode change writes nothing to the device */
+  const float on = gsbn_logf(1 + _eps), off = gsbn_logf(0 + _eps);
+  const int nrows = stim.data_rows;
+  if (nrows > 0 && stim.data_cols < (int)_mcu_in_hcu.size())
+    throw Error(-2, "Bad stimuli!!! (fewer columns than hypercolumns)");
+  auto* vl = _lginp->mutable_cpu_vector();
+  vl->assign((size_t)std::max(nrows, 1) * mcu_num, nrows > 0 ? off : 0.0f);
+  for (int r = 0; r < nrows; ++r) {
+    const uint32_t* lg = stim.data.data() + (size_t)r * stim.data_cols;
+    float* out = vl->data() + (size_t)r * mcu_num;
+    for (size_t i = 0; i < _mcu_in_hcu.size(); ++i)
+      for (int j = 0; j < _mcu_in_hcu[i]; ++j) *out++ = ((int32_t)lg[i] == j) ? on : off;
+  }
+  _lginp->resize(vl->size());
+  _lginp->set_ld(mcu_num);
   _cursor = 0;
-  _old_lgidx_id = -1;
 }
 
 void ProcExtGen::update_cpu() {
@@ -140,22 +155,10 @@ void ProcExtGen::update_cpu() {
       _glv.putf("prn", m.prn);
       _glv.puti("wmask-idx", m.wmask_id);
       _glv.putb("plasticity", (bool)m.plasticity);
-      _glv.puti("lginp-idx", 0);
+      if (m.lgidx_id < 0 || (size_t)(m.lgidx_id + 1) * _lgidx->ld() > _lgidx->size())
+        throw Error(-2, "stimulus row " + std::to_string(m.lgidx_id) + " out of range");
+      _glv.puti("lginp-idx", m.lgidx_id);
       _glv.puti("cycle-flag", 1);
-      if (_old_lgidx_id != m.lgidx_id) {
-        _old_lgidx_id = m.lgidx_id;
-        if ((size_t)(m.lgidx_id + 1) * _lgidx->ld() > _lgidx->size())
-          throw Error(-2, "stimulus row " + std::to_string(m.lgidx_id) + " out of range");
-        const int* lgidx = _lgidx->cpu_data(m.lgidx_id);
-        float* lginp = _lginp->mutable_cpu_data();
-        const float on = gsbn_logf(1 + _eps), off = gsbn_logf(0 + _eps);
-        for (size_t i = 0; i < _mcu_in_hcu.size(); i++) {
-          for (int j = 0; j < _mcu_in_hcu[i]; j++) {
-            *lginp = (lgidx[i] == j) ? on : off;
-            lginp++;
-          }
-        }
-      }
       return;
     } else if (current_step <= m.begin_step) {
       _glv.puti("cycle-flag", 0);
