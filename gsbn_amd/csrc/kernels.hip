@@ -3,11 +3,15 @@
  * kernel-level C ABI (include/gsbn_amd.h).
  *
  * Per executed step the library launches, per population, one fused
- * population kernel (one workgroup per hypercolumn) and, per projection, one
- * row/column kernel over a fixed grid that walks the step's active rows and
- * active columns; the delay-queue push is a one-workgroup compaction.  No
- * kernel allocates memory or needs a host round trip: every list length
- * stays on the device.
+ * population kernel (one workgroup per hypercolumn, which also pushes the
+ * step's spikes into the delay queues) and, per projection, one row/column
+ * kernel over a fixed grid that walks the step's active rows and active
+ * columns; or one launch per step instead: the fused step kernel (a
+ * hypercolumn's population and row/column update in one workgroup) or the
+ * chained step kernel (the previous step's row and column updates with this
+ * step's population update).  Batches of networks use the same kernels over
+ * workgroup maps.  No kernel allocates memory or needs a host round trip:
+ * every list length stays on the device.
  *
  * Determinism: every floating-point result is produced by the same sequence
  * of IEEE operations as the reference's sequential CPU loops
