@@ -135,6 +135,10 @@ class NetSpec:
     # row/column updates with this step's population update; -1 never;
     # 0: one of the plans chosen online
     chained_step: int = 0
+    # False: ProcExtGen builds one stimulus row per mode change into a
+    # one-row .lginp instead of keeping every row resident (argi
+    # resident-stimulus 0); the form it falls back to beyond 2^28 floats
+    resident_stimulus: bool = True
     # rowcol_graded g (1-3): batches split their row/column work per size
     # bucket of hypercolumns (plan -10-g); 0: chosen online
     rowcol_graded: int = 0
@@ -282,7 +286,10 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append("  }")
     L.append("}")
     L.append('rec_param { directory: "%s" }' % rec_dir)
-    L.append('proc_param { name: "ProcExtGen" }')
+    if spec.resident_stimulus:
+        L.append('proc_param { name: "ProcExtGen" }')
+    else:
+        L.append('proc_param { name: "ProcExtGen" argi { key: "resident-stimulus" val: 0 } }')
     L.append("proc_param {")
     L.append('  name: "ProcUpdLazy"')
     L.append('  argi { key: "seed-lo" val: %d }' % (spec.seed & 0x7FFFFFFF))

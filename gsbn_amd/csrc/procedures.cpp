@@ -54,6 +54,12 @@ class ProcExtGen : public ProcedureBase {
   vector<int> _mcu_in_hcu;
   float _eps = 0;
   int _cursor = 0;
+  /* every row's input resident in .lginp (selected by lginp-idx), or one
+   * row rebuilt per mode change when that would exceed kResidentMax floats */
+  static constexpr int64_t kResidentMax = (int64_t)1 << 28; /* 1 GiB */
+  bool _resident = true;
+  int _built_row = -1;
+  void build_row(const uint32_t* lg, float* out) const;
 };
 REGISTERIMPL(ProcExtGen);
 
@@ -123,21 +129,33 @@ void ProcExtGen::init_new(SolverParam solver_param, Database& db) {
    * the MCU the row names in each HCU, log(eps) elsewhere), built once: a
    * mode selects its row with lginp-idx, as .wmask's rows are selected by
    * wmask-idx, so a mode change writes nothing to the device */
-  const float on = gsbn_logf(1 + _eps), off = gsbn_logf(0 + _eps);
   const int nrows = stim.data_rows;
   if (nrows > 0 && stim.data_cols < (int)_mcu_in_hcu.size())
     throw Error(-2, "Bad stimuli!!! (fewer columns than hypercolumns)");
+  /* beyond kResidentMax floats the rows are built one at a time instead, at
+   * each mode change, into a one-row .lginp (the reference's way) */
+  _resident = (int64_t)std::max(nrows, 1) * mcu_num <= kResidentMax;
+  /* argi resident-stimulus 0 asks for the one-row form at any size */
+  for (auto& pp : solver_param.proc_param)
+    if (pp.name == proc_name()) {
+      int32_t v = 1;
+      if (Parser(pp).argi("resident-stimulus", v) && v == 0) _resident = false;
+    }
   auto* vl = _lginp->mutable_cpu_vector();
-  vl->assign((size_t)std::max(nrows, 1) * mcu_num, nrows > 0 ? off : 0.0f);
-  for (int r = 0; r < nrows; ++r) {
-    const uint32_t* lg = stim.data.data() + (size_t)r * stim.data_cols;
-    float* out = vl->data() + (size_t)r * mcu_num;
-    for (size_t i = 0; i < _mcu_in_hcu.size(); ++i)
-      for (int j = 0; j < _mcu_in_hcu[i]; ++j) *out++ = ((int32_t)lg[i] == j) ? on : off;
-  }
+  vl->assign((size_t)(_resident ? std::max(nrows, 1) : 1) * mcu_num, 0.0f);
+  if (_resident)
+    for (int r = 0; r < nrows; ++r) build_row(stim.data.data() + (size_t)r * stim.data_cols,
+                                             vl->data() + (size_t)r * mcu_num);
   _lginp->resize(vl->size());
   _lginp->set_ld(mcu_num);
   _cursor = 0;
+  _built_row = -1;
+}
+
+void ProcExtGen::build_row(const uint32_t* lg, float* out) const {
+  const float on = gsbn_logf(1 + _eps), off = gsbn_logf(0 + _eps);
+  for (size_t i = 0; i < _mcu_in_hcu.size(); ++i)
+    for (int j = 0; j < _mcu_in_hcu[i]; ++j) *out++ = ((int32_t)lg[i] == j) ? on : off;
 }
 
 void ProcExtGen::update_cpu() {
@@ -157,7 +175,16 @@ void ProcExtGen::update_cpu() {
       _glv.putb("plasticity", (bool)m.plasticity);
       if (m.lgidx_id < 0 || (size_t)(m.lgidx_id + 1) * _lgidx->ld() > _lgidx->size())
         throw Error(-2, "stimulus row " + std::to_string(m.lgidx_id) + " out of range");
-      _glv.puti("lginp-idx", m.lgidx_id);
+      if (_resident) {
+        _glv.puti("lginp-idx", m.lgidx_id);
+      } else {
+        if (_built_row != m.lgidx_id) {
+          _built_row = m.lgidx_id;
+          build_row(reinterpret_cast<const uint32_t*>(_lgidx->cpu_data(m.lgidx_id)),
+                    _lginp->mutable_cpu_data());
+        }
+        _glv.puti("lginp-idx", 0);
+      }
       _glv.puti("cycle-flag", 1);
       return;
     } else if (current_step <= m.begin_step) {

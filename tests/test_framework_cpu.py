@@ -116,7 +116,8 @@ def test_solver_without_device_fails_loudly(tmp_path):
                                              ("chained_step", "chained-step", -1),
                                              ("rowcol_graded", "rowcol-graded", 2),
                                              ("fused_step", "fused-step", 1),
-                                             ("rowcol_parts", "rowcol-parts", 15)])
+                                             ("rowcol_parts", "rowcol-parts", 15),
+                                             ("resident_stimulus", "resident-stimulus", 0)])
 def test_step_plan_keys_written(tmp_path, field, key, value):
     # the ProcUpdLazy plan options reach the prototxt, and the file still parses
     spec = configs.config0()

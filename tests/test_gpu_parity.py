@@ -61,6 +61,19 @@ def _lockstep_and_compare(spec, tmp_path, max_steps=None):
     return gpu, ora
 
 
+@pytest.mark.parametrize("chained", [-1, 1])
+def test_config0_per_mode_stimulus_rows(tmp_path, chained):
+    # ProcExtGen's one-row .lginp, rebuilt at each mode change (the form it
+    # takes beyond 2^28 resident floats), on the split and the chained plan
+    spec = configs.config0()
+    spec.resident_stimulus = False
+    spec.chained_step = chained
+    gpu, ora = _lockstep_and_compare(spec, tmp_path, max_steps=900)
+    assert gpu.vector(".lginp").size == spec.total_units
+    assert gpu.glob("lginp-idx") == 0
+    gpu.close()
+
+
 def test_config0_queue_kernel_path(tmp_path):
     # ProcExchangeSpike copies spikes into si_<proj> and the queue push runs
     # as its own kernel (gsbn_proj_que); same results as the direct path
