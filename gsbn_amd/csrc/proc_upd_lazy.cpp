@@ -1526,7 +1526,15 @@ void ProcUpdLazy::update_gpu() {
       op_check(gsbn_i32_fill(q->act_stamp1.p, R, -1, stream), "fill");
       int32_t res = 0;
       if (gsbn_chained_resident(q->prec, &res) != 0 || res <= 0) res = 768;
-      _chain_parts = std::max(1, std::min(32, (res - H) / std::max(H, 1)));
+      /* column workgroups per HCU: the workgroups of HCUs without a spike
+       * return at once, so slices finer than one dispatch round holds
+       * shorten the busy steps; about 128 slots per workgroup, at most 64
+       * parts and about 16 k column workgroups, and no fewer than one round
+       * holds (up to 32) */
+      const int fit = std::min(32, (res - H) / std::max(H, 1));
+      const int by_slots = (q->_dim_conn + 127) / 128;
+      const int budget = std::max(1, 16384 / std::max(H, 1));
+      _chain_parts = std::max(1, std::min(64, std::max(fit, std::min(by_slots, budget))));
       q->flush_hook = [this]() { flush_deferred(); };
     }
     if (_graded_fixed && !_batch)
