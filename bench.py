@@ -88,18 +88,46 @@ def apply_plan(spec, plan: str):
 
 
 def cpu_baseline(args) -> dict:
-    """Run the oracle in a child process pinned to one core."""
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-only", "--workload", args.workload,
-           "--warmup", str(args.warmup), "--cpu-seconds", str(args.cpu_seconds)]
-    out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
-                         timeout=900)
-    if out.returncode != 0:
-        return {"value": None, "unit": "synapse-trace updates/s", "cores": 1, "kind": "port",
-                "sample": "failed: " + out.stderr.strip().splitlines()[-1][:200]}
-    return json.loads(out.stdout.strip().splitlines()[-1])
+    """Run the oracle in child processes pinned to one core: first with the
+    reference's own float closed-form arithmetic (Proj.cpp:241-277, libm
+    exp per synapse update), the baseline, then with this project's
+    coefficient-table arithmetic, reported beside it."""
+    def child(arith, seconds):
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-only", "--workload",
+               args.workload, "--warmup", str(args.warmup), "--steps", str(args.steps),
+               "--cpu-seconds", str(seconds), "--cpu-arith", arith]
+        out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                             timeout=900)
+        if out.returncode != 0:
+            return {"value": None, "unit": "synapse-trace updates/s", "cores": 1, "kind": "port",
+                    "sample": "failed: " + (out.stderr.strip().splitlines() or ["?"])[-1][:200]}
+        return json.loads(out.stdout.strip().splitlines()[-1])
+    base = child("ref32", args.cpu_seconds)
+    if args.cpu_seconds > 0:
+        base["table_arith"] = child("table", args.cpu_seconds / 2)
+    return base
+
+
+def sweep_steps(spec, upto: int):
+    """Executed steps at which the learning rate changes (a full catch-up
+    sweep), from the schedule alone."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    eg = O.ExtGen(spec)
+    out = []
+    for _ in range(upto):
+        eg.update()
+        if eg.cycle_flag < 0:
+            break
+        if eg.cycle_flag == 1 and eg.prn != eg.old_prn:
+            out.append(eg.simstep)
+    return out
 
 
 def cpu_only(args):
+    """The oracle on one core: `warmup` steps, then the GPU's timed window
+    (steps warmup+1 .. warmup+steps) and on, without crossing a full sweep,
+    until about --cpu-seconds have passed."""
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except (AttributeError, OSError):
@@ -107,55 +135,134 @@ def cpu_only(args):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     spec, _ = workload(args.workload)
-    net = O.OracleNet(spec)
+    arith = {"ref32": O.ARITH_REF32, "table": O.ARITH_TABLE}[args.cpu_arith]
+    net = O.OracleNet(spec, arith)
     for _ in range(args.warmup):
         net.step()
+    sweeps = [t for t in sweep_steps(spec, args.warmup + 200000) if t > args.warmup]
+    stop = sweeps[0] if sweeps else None
     w0 = net.work()
     t0 = time.perf_counter()
-    n = 0
-    while time.perf_counter() - t0 < args.cpu_seconds:
+    n, win = 0, None
+    while True:
+        step = args.warmup + n + 1
+        if stop is not None and step >= stop:
+            break
+        if n >= args.steps and time.perf_counter() - t0 >= args.cpu_seconds:
+            break
         if net.step() < 0:
             break
         n += 1
+        if n == args.steps:
+            tw = time.perf_counter() - t0
+            ww = net.work()
+            win = {"steps": n, "seconds": tw,
+                   "value": float(np.sum(ww[:3] - w0[:3])) / tw, "steps_per_sec": n / tw}
     dt = time.perf_counter() - t0
     w1 = net.work()
-    upd = int((w1[0] - w0[0]) + (w1[1] - w0[1]) + (w1[2] - w0[2]))
+    upd = int(np.sum(w1[:3] - w0[:3]))
+    desc = {"ref32": "the reference's float closed forms with libm exp/log per update "
+                     "(Proj.cpp:241-277)",
+            "table": "this project's FP64-built coefficient table, float FMA-free updates"}
     print(json.dumps({"value": upd / dt, "unit": "synapse-trace updates/s", "cores": 1,
-                      "kind": "port", "steps_per_sec": n / dt,
-                      "sample": "%d steps after %d warm-up steps of the same workload, %.1f s, "
-                                "1 thread pinned to one core" % (n, args.warmup, dt)}))
+                      "kind": "port", "steps_per_sec": n / dt, "arith": desc[args.cpu_arith],
+                      "same_window": win,
+                      "sample": "steps %d..%d of the same workload (the GPU's timed steps "
+                                "%d..%d and the training steps after them, no full sweep), "
+                                "%.1f s, 1 thread pinned to one core"
+                                % (args.warmup + 1, args.warmup + n, args.warmup + 1,
+                                   args.warmup + args.steps, dt)}))
+
+
+PLAN_NAMES = {-1: "fused", -2: "chained"}
+
+
+def plan_name(parts: int) -> str:
+    if parts <= -10:
+        return "graded-%d" % (-10 - parts)
+    return PLAN_NAMES.get(parts, "split:%d" % parts)
+
+
+def fixed_plan(spec) -> bool:
+    return bool(spec.fused_step or spec.chained_step > 0 or spec.rowcol_parts or
+                spec.rowcol_graded)
+
+
+def fix_plan(spec, parts: int):
+    """Fix the step plan named by a GlobalVar rowcol-parts value."""
+    if parts == -1:
+        spec.fused_step = 1
+    elif parts == -2:
+        spec.chained_step = 1
+    elif parts <= -10:
+        spec.rowcol_graded = -10 - parts
+        spec.chained_step = -1
+    else:
+        spec.rowcol_parts = parts
+        spec.chained_step = -1
+
+
+def settle_plan(spec, max_steps: int = 5000):
+    """Run the online plan tuner to completion on a solver of its own (same
+    network, same schedule) and return (rowcol-parts it chose, steps it
+    took), so that the measured solvers run that plan from their first step
+    and no tuning sample falls inside a timed window."""
+    from gsbn_amd import Solver, configs
+    with tempfile.TemporaryDirectory() as d:
+        s = Solver(configs.write(spec, d))
+        configs.apply_conn_arrays(s, spec)
+        n = 0
+        while n < max_steps:
+            if s.step(1) == 0:
+                break
+            n += 1
+            if s.glob("plan-tuned"):
+                break
+        parts = s.glob("rowcol-parts")
+        tuned = s.glob("plan-tuned")
+        s.close()
+    if not tuned:
+        raise SystemExit("bench: the plan tuner did not finish within %d steps" % n)
+    return parts, n
+
+
+def _solver(spec, d):
+    from gsbn_amd import Solver, configs
+    t = time.perf_counter()
+    s = Solver(configs.write(spec, d))
+    configs.apply_conn_arrays(s, spec)
+    s.sync()
+    return s, time.perf_counter() - t
 
 
 def measure(spec, steps, warmup, step_times=0):
-    """Build the solver, run `warmup` steps, time `steps` steps; HIP events
-    around every row/column launch of the timed region."""
-    from gsbn_amd import Solver, configs
+    """Run `warmup` steps, then time `steps` steps with a fixed step plan
+    (the one the online tuner chooses for this workload, found beforehand on
+    a separate solver, unless the spec fixes one).
+
+    Kernel times come from a second solver of the same spec and plan that
+    replays the same steps (every result is deterministic) with HIP events
+    recorded by each launch of the row/column kernel, so the timed steps run
+    uninstrumented and the kernel figures cover exactly the launches of the
+    timed window.  In the chained plan the launch of step x runs the row and
+    column updates of step x-1; the work of a window is counted as what its
+    launches execute, in both solvers' figures."""
+    import copy
+    spec = copy.deepcopy(spec)
+    tune_steps = 0
+    if not fixed_plan(spec):
+        parts, tune_steps = settle_plan(spec)
+        fix_plan(spec, parts)
     with tempfile.TemporaryDirectory() as d:
-        path = configs.write(spec, d)
-        t_init = time.perf_counter()
-        s = Solver(path)
-        configs.apply_conn_arrays(s, spec)
-        s.sync()
-        t_init = time.perf_counter() - t_init
+        s, t_init = _solver(spec, d)
+        parts = s.glob("rowcol-parts")
+        lag = 1 if parts == -2 else 0
         s.step(warmup)
         s.sync()
-        w0 = s.work()
         t0 = time.perf_counter()
         done = s.step(steps)
         s.sync()
         t1 = time.perf_counter()
-        w1 = s.work()
-        parts = s.glob("rowcol-parts")
-        # kernel times in a second pass over the following steps: HIP events
-        # recorded by every row/column launch (left out of the timed steps,
-        # which run without instrumentation)
-        s.timing(-1, collect=False)
-        k0 = s.work()
-        s.timing((1 << 1) | (1 << 6), collect=False)
-        s.step(min(steps, 1000))
-        s.sync()
-        launches, ms = s.timing(0)
-        k1 = s.work()
         step_ms = None
         if step_times:
             # per-step wall times (each step synchronised) for the p50/p99 report
@@ -168,19 +275,36 @@ def measure(spec, steps, warmup, step_times=0):
             step_ms = {"p50": float(np.percentile(ts, 50)), "p99": float(np.percentile(ts, 99)),
                        "n": len(ts)}
         s.close()
+    with tempfile.TemporaryDirectory() as d:
+        r, _ = _solver(spec, d)
+        r.step(warmup - lag)
+        k0 = r.work()
+        r.step(lag)
+        r.timing(-1, collect=False)
+        r.timing((1 << 1) | (1 << 6), collect=False)
+        r.step(steps - lag)
+        k1 = r.work()
+        r.step(lag)
+        r.sync()
+        launches, ms = r.timing(0)
+        r.timing(-1, collect=False)
+        r.close()
     el = t1 - t0
-    rows = int(w1[0] - w0[0])
-    cols = int(w1[1] - w0[1])
-    sweep = int(w1[2] - w0[2])
+    rows = int(k1[0] - k0[0])
+    cols = int(k1[1] - k0[1])
+    sweep = int(k1[2] - k0[2])
     bpu = BYTES_PER_SYN_UPDATE[spec.trace_precision]
     # one launch per step (the fused plan, -1, or the chained plan, -2) is
     # timed as kind 6; split plans time their row/column launches (kind 1)
-    fused = int(launches[6]) > int(launches[1])
-    kind = 6 if fused else 1
+    one_launch = parts in (-1, -2)
+    kind = 6 if one_launch else 1
     rc_launch, rc_ms = int(launches[kind]), float(ms[kind])
+    other = int(launches[1]) if one_launch else int(launches[6])
+    if other:
+        raise SystemExit("bench: %d launches of another plan in the kernel window" % other)
     avg_ms = rc_ms / max(rc_launch, 1)
-    bytes_per_launch = bpu * int((k1[0] - k0[0]) + (k1[1] - k0[1])) / max(rc_launch, 1)
-    if fused:
+    bytes_per_launch = bpu * (rows + cols) / max(rc_launch, 1)
+    if one_launch:
         bytes_per_launch += POP_BYTES_PER_UNIT * spec.total_units
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return {"elapsed": el, "done": int(done), "rows": rows, "cols": cols, "sweep": sweep,
@@ -188,10 +312,11 @@ def measure(spec, steps, warmup, step_times=0):
             "ms_per_step": 1e3 * el / max(done, 1), "init_s": t_init, "step_ms": step_ms,
             "rc_launches": rc_launch, "rc_avg_ms": avg_ms, "rc_bytes_per_launch": bytes_per_launch,
             "rc_achieved_gbs": achieved, "rc_parts": parts if parts > 0 else None,
-            "rc_plan": ({-1: "fused", -2: "chained"}.get(parts, "split") if parts > -10
-                        else "graded-%d" % (-10 - parts)),
-            "rc_kernel": ("chained_kernel" if parts == -2 else "step_kernel") if fused
-            else "rowcol_kernel"}
+            "rc_plan": plan_name(parts), "tune_steps": tune_steps,
+            "rc_kernel": ("chained_kernel" if parts == -2 else "step_kernel") if one_launch
+            else "rowcol_kernel",
+            "kernel_window": "steps %d..%d, replayed with events on every launch"
+                             % (warmup + 1, warmup + steps)}
 
 
 def copy_calibration(mib: int):
@@ -376,6 +501,7 @@ def main():
                     help="step plan of the single-network runs: auto (tuned online), "
                          "split:N (N row/column workgroups per HCU), fused or chained")
     ap.add_argument("--cpu-only", action="store_true")
+    ap.add_argument("--cpu-arith", default="ref32", choices=("ref32", "table"))
     ap.add_argument("--step-times", type=int, default=0,
                     help="after the timed region, time this many synchronised single steps")
     args = ap.parse_args()
@@ -391,15 +517,18 @@ def main():
     spec, desc = workload(args.workload)
     apply_plan(spec, args.plan)
     m = measure(spec, args.steps, args.warmup, args.step_times)
-    # HBM bytes per launch of the timed kernel, from the rocprofv3 counter
-    # runs recorded for this workload (per kernel)
-    traffic = None
+    # HBM bytes per launch of the timed kernel: the ratio of counter bytes
+    # (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy of known size)
+    # to algorithmic bytes that the counter runs of this kernel and plan
+    # measured, applied to this run's algorithmic bytes per launch
+    traffic, traffic_ratio = None, None
     tp = os.path.join(ROOT, "profiles", "rowcol_%s_traffic.json" % args.workload)
     if os.path.exists(tp):
         with open(tp) as f:
             rec = json.load(f).get("kernels", {}).get(m["rc_kernel"])
-        if rec:
-            traffic = rec.get("hbm_bytes_per_launch")
+        if rec and rec.get("plan") == m["rc_plan"] and rec.get("algorithmic_bytes_per_launch"):
+            traffic_ratio = rec["hbm_bytes_per_launch"] / rec["algorithmic_bytes_per_launch"]
+            traffic = int(round(traffic_ratio * m["rc_bytes_per_launch"]))
     achieved = m["rc_achieved_gbs"]
     line = {
         "metric": METRIC,
@@ -425,9 +554,12 @@ def main():
         "init_s": m["init_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_ratio": traffic_ratio,
                      "kernel": m["rc_kernel"], "avg_launch_ms": m["rc_avg_ms"],
                      "launches": m["rc_launches"], "bytes_per_launch": m["rc_bytes_per_launch"],
-                     "workgroups_per_hcu": m["rc_parts"], "step_plan": m["rc_plan"]},
+                     "window": m["kernel_window"],
+                     "workgroups_per_hcu": m["rc_parts"], "step_plan": m["rc_plan"],
+                     "plan_tuning_steps": m["tune_steps"]},
     }
     if m["step_ms"] is not None:
         line["step_ms"] = m["step_ms"]

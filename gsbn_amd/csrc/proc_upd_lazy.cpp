@@ -1281,6 +1281,8 @@ void ProcUpdLazy::finish_tuning() {
   }
   _tuned = true;
   _glv.puti("rowcol-parts", _parts);
+  /* the plan is final from the next step on */
+  _glv.puti("plan-tuned", 1);
 }
 
 /* init_new (ProcUpdLazy.cpp:8-46) and init_copy (ProcUpdLazy.cpp:48-86) */
@@ -1544,6 +1546,9 @@ void ProcUpdLazy::update_gpu() {
                               : _chain_fixed > 0 ? -2
                               : _graded_fixed > 0 ? -10 - _graded_fixed
                               : _parts_fixed > 0 ? _parts_fixed : _parts);
+    /* GlobalVar plan-tuned: 1 once the plan no longer changes */
+    _glv.puti("plan-tuned", (_fused_fixed || _chain_fixed > 0 || _graded_fixed > 0 ||
+                             _parts_fixed > 0 || _list_proj.empty()) ? 1 : 0);
     _configured = true;
     _args_dirty = true;
   }

@@ -281,6 +281,7 @@ typedef struct {
   int p_shift, e_shift;    /* binary16 form: x is stored as binary16(x * 2^shift) */
   int ncache;
   gor_coefcache_t cache[GOR_MAXCACHE];
+  int mutation;
 } gor_proj_t;
 
 struct gor_net {
@@ -297,6 +298,7 @@ struct gor_net {
   int wmask_rows;
   int total_units, total_hcus;
   int64_t work[4];
+  int mutation; /* gor_set_mutation: a planted mistake, 0 = none */
 };
 
 static size_t tsize(const gor_net_t* n) { return n->arith == GOR_ARITH_F64 ? sizeof(double) : sizeof(float); }
@@ -772,14 +774,13 @@ static void pop_kernel_1(gor_net_t* n, gor_pop_t* p, int i, int j, const float* 
   int mcu_num_in_pop = p->H * p->M;
   int m;
   for (m = 0; m < p->dim_proj; m++) {
-    wsup += bj[offset + idx] + epsc[offset + idx];
+    wsup += (n->mutation == GOR_MUT_SUP_NO_BJ ? 0.0f : bj[offset + idx]) + epsc[offset + idx];
     offset += mcu_num_in_pop;
   }
   float sup = p->lgbias + p->igain * lginp[idx] + p->rnd_n[idx];
   sup += (p->wgain * wmask[i]) * wsup;
   float dsup = p->dsup[idx];
   p->dsup[idx] += (sup - dsup) * p->taumdt;
-  (void)n;
 }
 
 /* update_sup_kernel_2_cpu, Pop.cpp:263-303; per-HCU maximum unless
@@ -806,9 +807,10 @@ static void pop_kernel_2(gor_net_t* n, gor_pop_t* p, int i) {
 }
 
 /* update_sup_kernel_3_cpu, Pop.cpp:305-323 */
-static void pop_kernel_3(gor_pop_t* p, int i, int j) {
+static void pop_kernel_3(gor_net_t* n, gor_pop_t* p, int i, int j) {
   int idx = i * p->M + j;
-  int8_t spk = (int8_t)(p->rnd_u[idx] < p->act[idx] * p->maxfqdt);
+  int8_t spk = n->mutation == GOR_MUT_SPIKE_NO_MAXFQ ? (int8_t)(p->rnd_u[idx] < p->act[idx])
+                                                     : (int8_t)(p->rnd_u[idx] < p->act[idx] * p->maxfqdt);
   p->spike[idx] = spk;
   p->counter[idx] += spk;
 }
@@ -829,7 +831,7 @@ static void pop_step(gor_net_t* n, gor_pop_t* p) {
     for (j = 0; j < p->M; j++)
       pop_kernel_1(n, p, i, j, lginp, wmask, (const float*)p->epsc, (const float*)p->bj);
     pop_kernel_2(n, p, i);
-    for (j = 0; j < p->M; j++) pop_kernel_3(p, i, j);
+    for (j = 0; j < p->M; j++) pop_kernel_3(n, p, i, j);
   }
   /* fill_spike (Pop.cpp:389-409) */
   if (p->ext_off) {
@@ -875,6 +877,12 @@ float gor_det_logf(float x) { return gsbn_logf(x); }
 float gor_det_expf(float x) { return gsbn_expf(x); }
 
 void gor_work_counters(gor_net_t* n, int64_t* out4) { memcpy(out4, n->work, sizeof(n->work)); }
+
+void gor_set_mutation(gor_net_t* n, int code) {
+  int i;
+  n->mutation = code;
+  for (i = 0; i < n->nproj; i++) n->projs[i].mutation = code;
+}
 
 /* ------------------------------------------------------------------ */
 /* vector access by Database name */

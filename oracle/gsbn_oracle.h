@@ -118,6 +118,26 @@ float gor_det_expf(float x);
  * synapse updates, full-sweep synapse updates, steps */
 void gor_work_counters(gor_net_t* n, int64_t* out4);
 
+/* Planted mistakes for testing the tests (default 0: none).  Each makes
+ * the oracle deviate from the reference in one plausible way, so that a pin
+ * that is meant to fix that part of the algorithm can be shown to fail:
+ *   SWEEP_NEW_PRN   full sweep with kp = taupdt * prn instead of old-prn
+ *   SWEEP_TARGET_T  full sweep to simstep instead of simstep - 1
+ *   SWEEP_W_KP0     full sweep recomputes Wij also when kp == 0
+ *   SWEEP_OLD_PI    full sweep's Wij of slot j == 0 uses pi before its catch-up
+ *   SUP_NO_BJ       support sum leaves out bj
+ *   SPIKE_NO_MAXFQ  spike when u < act instead of u < act * maxfq * dt */
+enum {
+  GOR_MUT_NONE = 0,
+  GOR_MUT_SWEEP_NEW_PRN = 1,
+  GOR_MUT_SWEEP_TARGET_T = 2,
+  GOR_MUT_SWEEP_W_KP0 = 3,
+  GOR_MUT_SWEEP_OLD_PI = 4,
+  GOR_MUT_SUP_NO_BJ = 5,
+  GOR_MUT_SPIKE_NO_MAXFQ = 6
+};
+void gor_set_mutation(gor_net_t* n, int code);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 ARITH_TABLE, ARITH_REF32, ARITH_F64 = 0, 1, 2
+# planted mistakes (gsbn_oracle.h GOR_MUT_*), for showing that a pin fails
+MUTATIONS = {"sweep_new_prn": 1, "sweep_target_t": 2, "sweep_w_kp0": 3, "sweep_old_pi": 4,
+             "sup_no_bj": 5, "spike_no_maxfq": 6}
 _DT = {0: np.int8, 2: np.int32, 5: np.float32, 6: np.float64}
 
 
@@ -87,6 +90,7 @@ def lib():
         L.gor_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.c_uint32, vp, vp]
         L.gor_work_counters.argtypes = [vp, vp]
+        L.gor_set_mutation.argtypes = [vp, i32]
         L.gor_det_logf.restype = f32
         L.gor_det_logf.argtypes = [f32]
         L.gor_det_expf.restype = f32
@@ -693,6 +697,11 @@ class OracleNet:
             if max_steps is not None and n >= max_steps:
                 break
         return raster
+
+    def plant(self, mistake: Optional[str]):
+        """Make this network deviate from the reference in one planted way
+        (MUTATIONS), or restore it (None); only tests of the tests use this."""
+        lib().gor_set_mutation(self.h, MUTATIONS[mistake] if mistake else 0)
 
     def work(self) -> np.ndarray:
         o = np.zeros(4, np.int64)
