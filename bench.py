@@ -226,6 +226,11 @@ def settle_plan(spec, max_steps: int = 5000):
     return parts, n
 
 
+def _steps(s, n):
+    if n > 0:
+        s.step(n)
+
+
 def _solver(spec, d):
     from gsbn_amd import Solver, configs
     t = time.perf_counter()
@@ -255,10 +260,12 @@ def measure(spec, steps, warmup, step_times=0):
         fix_plan(spec, parts)
     with tempfile.TemporaryDirectory() as d:
         s, t_init = _solver(spec, d)
-        parts = s.glob("rowcol-parts")
-        lag = 1 if parts == -2 else 0
-        s.step(warmup)
+        lag = 1 if spec.chained_step > 0 else 0
+        if warmup < 1 or steps < 1:
+            raise SystemExit("bench: needs --warmup >= 1 and --steps >= 1")
+        _steps(s, warmup)
         s.sync()
+        parts = s.glob("rowcol-parts")
         t0 = time.perf_counter()
         done = s.step(steps)
         s.sync()
@@ -277,14 +284,14 @@ def measure(spec, steps, warmup, step_times=0):
         s.close()
     with tempfile.TemporaryDirectory() as d:
         r, _ = _solver(spec, d)
-        r.step(warmup - lag)
+        _steps(r, warmup - lag)
         k0 = r.work()
-        r.step(lag)
+        _steps(r, lag)
         r.timing(-1, collect=False)
         r.timing((1 << 1) | (1 << 6), collect=False)
-        r.step(steps - lag)
+        _steps(r, steps - lag)
         k1 = r.work()
-        r.step(lag)
+        _steps(r, lag)
         r.sync()
         launches, ms = r.timing(0)
         r.timing(-1, collect=False)

@@ -878,6 +878,12 @@ float gor_det_expf(float x) { return gsbn_expf(x); }
 
 void gor_work_counters(gor_net_t* n, int64_t* out4) { memcpy(out4, n->work, sizeof(n->work)); }
 
+int gor_set_pop_id(gor_net_t* n, int pop, int id) {
+  if (pop < 0 || pop >= n->npop || id < 0) return -1;
+  n->pops[pop].id = id;
+  return 0;
+}
+
 void gor_set_mutation(gor_net_t* n, int code) {
   int i;
   n->mutation = code;

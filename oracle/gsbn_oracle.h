@@ -118,6 +118,11 @@ float gor_det_expf(float x);
  * synapse updates, full-sweep synapse updates, steps */
 void gor_work_counters(gor_net_t* n, int64_t* out4);
 
+/* the population id that keys the population's random draws (default: its
+ * index); a network that holds a few populations of a larger batch uses
+ * their ids in the batch so that their draws are the batch's */
+int gor_set_pop_id(gor_net_t* n, int pop, int id);
+
 /* Planted mistakes for testing the tests (default 0: none).  Each makes
  * the oracle deviate from the reference in one plausible way, so that a pin
  * that is meant to fix that part of the algorithm can be shown to fail:

@@ -91,6 +91,8 @@ def lib():
                                ctypes.c_uint32, vp, vp]
         L.gor_work_counters.argtypes = [vp, vp]
         L.gor_set_mutation.argtypes = [vp, i32]
+        L.gor_set_pop_id.restype = i32
+        L.gor_set_pop_id.argtypes = [vp, i32, i32]
         L.gor_det_logf.restype = f32
         L.gor_det_logf.argtypes = [f32]
         L.gor_det_expf.restype = f32
@@ -584,6 +586,9 @@ class OracleNet:
                                1 if q.tauepsc is not None else 0, q.slot_num)
         self.h = L.gor_net_new(spec.dt, len(spec.pops), pops, len(spec.projs), projs,
                                spec.seed, spec.wta_pop_max, arith)
+        for i, pid in enumerate(getattr(spec, "pop_ids", None) or []):
+            if L.gor_set_pop_id(self.h, i, pid) != 0:
+                raise ValueError("oracle: bad population id")
         self.arith = arith
         self.extgen = ExtGen(spec)
         self.check = Check(spec, spec.check_threshold)

@@ -73,3 +73,43 @@ def run_lockstep(spec, tmp_path, max_steps=None, check_every=0):
         if f < 0 or (max_steps is not None and steps >= max_steps):
             break
     return gpu, ora, diverged, steps
+
+
+def subset(spec, nets):
+    """The networks `nets` of a batch (network i = population i with its
+    recurrent projection i) as a NetSpec of their own, in that order; the
+    oracle keys their draws by their ids in the batch (pop_ids), so they run
+    as they do inside the batch."""
+    import copy
+    import dataclasses
+    starts = np.cumsum([0] + [p.hcu for p in spec.pops])
+    cols = np.concatenate([np.arange(starts[i], starts[i + 1]) for i in nets])
+    pops = [copy.deepcopy(spec.pops[i]) for i in nets]
+    projs = [dataclasses.replace(spec.projs[i], src=k, dest=k) for k, i in enumerate(nets)]
+    assert all(spec.projs[i].src == i and spec.projs[i].dest == i for i in nets)
+    sub = dataclasses.replace(spec, pops=pops, projs=projs, stim=spec.stim[:, cols],
+                              mask=spec.mask[:, cols], conn=[[] for _ in nets],
+                              conn_arrays=[spec.conn_arrays[i] for i in nets])
+    sub.pop_ids = list(nets)
+    return sub
+
+
+def compare_nets(gpu, ora, spec, nets, names=None):
+    """compare_state for networks `nets` of a batch on the product side
+    against networks 0.. of their subset on the oracle side."""
+    bad = []
+    for k, i in enumerate(nets):
+        for base in (names or POP_VECTORS + PROJ_VECTORS):
+            a = gpu.vector("%s_%d" % (base, i))
+            b = ora.vector("%s_%d" % (base, k))
+            if a.shape != b.shape:
+                bad.append((i, base, "shape %s vs %s" % (a.shape, b.shape)))
+                continue
+            if a.dtype.kind == "f":
+                ne = a.view(np.uint32) != b.view(np.uint32)
+            else:
+                ne = a != b
+            if ne.any():
+                j = int(np.nonzero(ne)[0][0])
+                bad.append((i, base, int(ne.sum()), "first at %d: gpu %r oracle %r" % (j, a[j], b[j])))
+    return bad

@@ -182,3 +182,82 @@ def test_batched_networks_mixed_block_sizes(tmp_path):
     c, t, flags = gpu.check()
     assert (c, t) == (ora.check.correct, ora.check.total)
     gpu.close()
+
+
+def _sweep_steps(spec, upto):
+    eg = O.ExtGen(spec)
+    out = []
+    for _ in range(upto):
+        eg.update()
+        if eg.cycle_flag == 1 and eg.prn != eg.old_prn:
+            out.append(eg.simstep)
+    return out
+
+
+def test_config1_through_first_recall_windows(tmp_path):
+    # 100 x 100 through the end of training (6000 steps) and the first two
+    # reset and recall windows: the learning rate switches at 6001 (1 ->
+    # 1e-30), 6051 (-> 0), 6101 (-> 1e-30) and 6151 (-> 0), each with a full
+    # sweep of 9.9e7 synapses over gaps of up to 6000 steps; spikes every
+    # step, every Database vector after each of these sweeps
+    spec = configs.config1()
+    sweeps = [t for t in _sweep_steps(spec, 6160) if t > 1]
+    assert sweeps == [6001, 6051, 6101, 6151]
+    path = helpers.write_spec(spec, tmp_path)
+    from gsbn_amd import Solver
+    gpu = Solver(path)
+    ora = O.OracleNet(spec)
+    compared = 0
+    while ora.extgen.simstep < 6160:
+        gpu.step(1)
+        assert ora.step() == 1
+        t = ora.extgen.simstep
+        assert np.array_equal(gpu.vector("spike_0"), ora.vector("spike_0")), \
+            "spikes differ at step %d" % t
+        if t in sweeps:
+            bad = helpers.compare_state(gpu, ora, spec)
+            assert not bad, (t, bad)
+            compared += 1
+    assert compared == 4
+    w = gpu.work()
+    assert w[2] == 5 * 100 * 9900 * 100  # five full sweeps, the first one at step 1
+    gpu.close()
+    ora.close()
+
+
+def test_batch512_sampled_networks_bitwise(tmp_path):
+    # the 512-network workload (HCU 8-64, MCU 16-128, p in {0.1, 0.5, 0.9},
+    # 2.66e9 synapses) in one solver; the smallest network, the largest one
+    # and a dense (p = 0.9) one of middle size run in the oracle as a network
+    # of their own, with their draws keyed by their ids in the batch.  Through
+    # the end of training (1200 steps) and the learning-rate switches at 1201
+    # and 1251: spikes every step, every Database vector of the three after
+    # each sweep
+    spec = configs.batched_nets(512)
+    syn = [p.hcu * int(q.slot_num) * p.mcu for p, q in zip(spec.pops, spec.projs)]
+    dens = [(ii >= 0).sum() / (p.hcu * (p.hcu - 1) * p.mcu)
+            for (ii, _), p in zip(spec.conn_arrays, spec.pops)]
+    small, large = int(np.argmin(syn)), int(np.argmax(syn))
+    dense = [int(i) for i in np.argsort(syn) if dens[i] > 0.8 and i not in (small, large)]
+    nets = [small, large, dense[len(dense) // 2]]
+    assert dens[nets[2]] > 0.8 and syn[large] > 5e7
+    sub = helpers.subset(spec, nets)
+    sweeps = _sweep_steps(spec, 1260)
+    assert sweeps == [1, 1201, 1251]
+    path = helpers.write_spec(spec, tmp_path)
+    from gsbn_amd import Solver
+    gpu = Solver(path)
+    configs.apply_conn_arrays(gpu, spec)
+    ora = O.OracleNet(sub)
+    while ora.extgen.simstep < 1260:
+        gpu.step(1)
+        assert ora.step() == 1
+        t = ora.extgen.simstep
+        for k, i in enumerate(nets):
+            assert np.array_equal(gpu.vector("spike_%d" % i), ora.vector("spike_%d" % k)), \
+                "network %d: spikes differ at step %d" % (i, t)
+        if t in sweeps[1:]:
+            bad = helpers.compare_nets(gpu, ora, spec, nets)
+            assert not bad, (t, bad[:4])
+    gpu.close()
+    ora.close()

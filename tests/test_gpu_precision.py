@@ -84,3 +84,41 @@ def test_batched_networks_bf16(tmp_path):
     c, t, flags = gpu.check()
     assert (c, t) == (ora.check.correct, ora.check.total)
     gpu.close()
+
+
+@pytest.mark.parametrize("point", ["taup5", "close"])
+@pytest.mark.parametrize("form", ["bf16", "fp16"])
+def test_config2_points_lockstep(tmp_path, point, form):
+    # the precision workload's two extra parameter points at 100 x 100: taup
+    # 5 s, and taup 5 s with taue = taup - 1 ms (the cancellation case); the
+    # first 120 steps through the first sweep, bit for bit
+    import precision_study as PS
+    spec = PS.point_spec(point, 100, 100)
+    spec.trace_precision = form
+    gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path, max_steps=120)
+    assert div is None, "spike rasters diverge at step %s" % (div,)
+    _check_shifts(gpu, ora, spec, form)
+    bad = helpers.compare_state(gpu, ora, spec)
+    assert not bad, bad
+    gpu.close()
+    ora.close()
+
+
+# error bounds against the exact double-precision replay at a short horizon
+# (300 steps of the 100 x 100 workload, product on the GPU); measured values
+# with margin: fp32 Pij max ~6e-7; bf16 ~2.2e-2 (its first rounding of the
+# initial 1/(C M) alone is 2.2e-3); fp16 ~2.6e-3
+SHORT_BOUNDS = {"fp32": (2e-6, 1e-5), "bf16": (5e-2, 5e-2), "fp16": (6e-3, 6e-3)}
+
+
+@pytest.mark.parametrize("point", ["default", "close"])
+@pytest.mark.parametrize("form", ["fp32", "bf16", "fp16"])
+def test_config2_error_against_exact_short(point, form):
+    import precision_study as PS
+    r = PS.run(form, point, 300, "gpu", 100, 100, log=lambda m: None)
+    pmax, wmax = SHORT_BOUNDS[form]
+    print("%s %s: Pij max %.2e mean %.2e, Wij abs max %.2e, bj max %.2e"
+          % (form, point, r["pij"]["max"], r["pij"]["mean"], r["wij"]["max"], r["bj"]["max"]))
+    assert r["pij"]["max"] <= pmax, r["pij"]
+    assert r["wij"]["max"] <= wmax, r["wij"]
+    assert r["bj"]["max"] <= 1e-5, r["bj"]

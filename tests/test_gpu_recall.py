@@ -152,3 +152,81 @@ def test_recall_mfma_agrees_with_exact(trained10):
     print("recall exact %.3f mfma %.3f outcome agreement %.3f winner agreement %.3f"
           % (ok_exact.mean(), ok_mfma.mean(), agree, winners))
     assert agree >= 0.9 and winners >= 0.9
+
+
+def test_spike_gemm_full_recall_shape_sampled_tiles():
+    # the support product at the frozen-recall shape of the 100 x 100
+    # network: (8192 cues x 10^4 units) 0/1 spikes times (10^4 x 10^4) bf16
+    # weights, checked on sampled 64 x 64 output tiles (corners, middle,
+    # ragged edge) against float64 within the float summation bound
+    rng = np.random.default_rng(21)
+    B, N, K = 8192, 10000, 10000
+    Kp = (K + 63) // 64 * 64
+    spk = rng.random((B, K)) < 0.01
+    S = np.zeros((B, Kp), np.uint16)
+    S[:, :K] = np.where(spk, 0x3F80, 0)
+    WT = np.zeros((N, Kp), np.uint16)
+    WT[:, :K] = _bf16_bits(rng.normal(0, 0.5, size=(N, K)).astype(np.float32))
+    E = rng.normal(0, 1, size=(B, N)).astype(np.float32)
+    beta = np.float32(0.95)
+    got = _gemm(S, WT, E, float(beta))
+    rows = [0, 4096, B - 64]
+    cols = [0, 5000, N - 64]
+    for r0 in rows:
+        sp = spk[r0:r0 + 64].astype(np.float64)
+        for c0 in cols:
+            Wb = (WT[c0:c0 + 64, :K].astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+            want = np.float64(beta) * E[r0:r0 + 64, c0:c0 + 64].astype(np.float64) + sp @ Wb.T
+            bound = 2 ** -22 * (np.abs(E[r0:r0 + 64, c0:c0 + 64]) + sp @ np.abs(Wb).T
+                                + 1e-30) * (K + 2)
+            err = np.abs(got[r0:r0 + 64, c0:c0 + 64].astype(np.float64) - want)
+            assert np.all(err <= bound), (r0, c0, float(err.max()))
+
+
+@pytest.fixture(scope="module")
+def trained100(tmp_path_factory):
+    spec = configs.config1()
+    s = _train(tmp_path_factory.mktemp("train100"), spec)
+    yield spec, s
+    s.close()
+
+
+def test_recall_8192_cues_exact_sampled_against_oracle(trained100):
+    # the frozen-recall workload at its own size: the 100 x 100 network
+    # trained through its 6000 training steps, 8192 cues (40 % of the HCUs
+    # clamped) x 50 lock-step steps in the exact mode; four sampled cues
+    # (first, second, middle, last) against single-cue oracle runs with the
+    # same frozen weights and draws: spikes of every step, winners, counts
+    spec, s = trained100
+    pats = spec.stim[:100].astype(np.int64)
+    B, T = 8192, 50
+    clamp, wmask, want = R.cues(pats, B, 0.4, seed=11)
+    rc = R.Recall.from_solver(s, spec, B, R.EXACT, seed=77)
+    idx, cnt, ras, _ = rc.run(clamp, wmask, T, count_from=0, raster=True)
+    rc.close()
+    ii, di = s.vector("ii_0"), s.vector("di_0")
+    wij, bj = s.vector("wij_0"), s.vector("bj_0")
+    H, M = spec.pops[0].hcu, spec.pops[0].mcu
+    spiking = 0
+    for b in (0, 1, B // 2, B - 1):
+        ospec = R.oracle_recall_spec(spec, clamp[b], wmask[b], T, 77 + b, ii, di)
+        net = O.OracleNet(ospec)
+        net.write("wij_0", wij)
+        net.write("bj_0", bj)
+        counts = np.zeros(H * M, np.int64)
+        for t in range(T):
+            assert net.step() == 1
+            sp = net.vector("spike_0")
+            assert np.array_equal(sp, ras[t, b]), "cue %d diverges at step %d" % (b, t + 1)
+            counts += sp
+        spiking += int(counts.sum())
+        c = counts.reshape(H, M)
+        for h in range(H):
+            mi, mc = -1, 0
+            for j in range(M):
+                if c[h, j] > mc:
+                    mi, mc = j, int(c[h, j])
+            assert (idx[b, h], cnt[b, h]) == (mi, mc)
+        net.close()
+    assert spiking > 0
+    del ras
