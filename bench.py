@@ -80,6 +80,9 @@ def apply_plan(spec, plan: str):
         spec.fused_step = 1
     elif plan == "chained":
         spec.chained_step = 1
+    elif plan == "wave":
+        spec.rowcol_wave = 1
+        spec.chained_step = -1
     elif plan.startswith("split:"):
         spec.rowcol_parts = int(plan.split(":", 1)[1])
         spec.chained_step = -1
@@ -174,18 +177,18 @@ def cpu_only(args):
                                    args.warmup + args.steps, dt)}))
 
 
-PLAN_NAMES = {-1: "fused", -2: "chained"}
+PLAN_NAMES = {-1: "fused", -2: "chained", -20: "wave"}
 
 
 def plan_name(parts: int) -> str:
-    if parts <= -10:
+    if -20 < parts <= -10:
         return "graded-%d" % (-10 - parts)
     return PLAN_NAMES.get(parts, "split:%d" % parts)
 
 
 def fixed_plan(spec) -> bool:
     return bool(spec.fused_step or spec.chained_step > 0 or spec.rowcol_parts or
-                spec.rowcol_graded)
+                spec.rowcol_graded or spec.rowcol_wave)
 
 
 def fix_plan(spec, parts: int):
@@ -194,6 +197,9 @@ def fix_plan(spec, parts: int):
         spec.fused_step = 1
     elif parts == -2:
         spec.chained_step = 1
+    elif parts == -20:
+        spec.rowcol_wave = 1
+        spec.chained_step = -1
     elif parts <= -10:
         spec.rowcol_graded = -10 - parts
         spec.chained_step = -1
@@ -321,7 +327,7 @@ def measure(spec, steps, warmup, step_times=0):
             "rc_achieved_gbs": achieved, "rc_parts": parts if parts > 0 else None,
             "rc_plan": plan_name(parts), "tune_steps": tune_steps,
             "rc_kernel": ("chained_kernel" if parts == -2 else "step_kernel") if one_launch
-            else "rowcol_kernel",
+            else "rowcol_kernel_batch_wave" if parts == -20 else "rowcol_kernel",
             "kernel_window": "steps %d..%d, replayed with events on every launch"
                              % (warmup + 1, warmup + steps)}
 
@@ -505,8 +511,8 @@ def main():
                     help="first copy MIB MiB five times with the library's copy kernel "
                          "(reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--plan", default="auto",
-                    help="step plan of the single-network runs: auto (tuned online), "
-                         "split:N (N row/column workgroups per HCU), fused or chained")
+                    help="step plan: auto (tuned online), split:N (N row/column workgroups "
+                         "per HCU), fused, chained, or wave (batches: one wave per HCU)")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--cpu-arith", default="ref32", choices=("ref32", "table"))
     ap.add_argument("--step-times", type=int, default=0,

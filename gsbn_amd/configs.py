@@ -142,6 +142,9 @@ class NetSpec:
     # rowcol_graded g (1-3): batches split their row/column work per size
     # bucket of hypercolumns (plan -10-g); 0: chosen online
     rowcol_graded: int = 0
+    # rowcol_wave 1: batches run their row/column update with one wave per
+    # hypercolumn (plan -20); 0: chosen online
+    rowcol_wave: int = 0
 
     @property
     def total_hcu(self) -> int:
@@ -307,6 +310,8 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('  argi { key: "chained-step" val: %d }' % spec.chained_step)
     if spec.rowcol_graded:
         L.append('  argi { key: "rowcol-graded" val: %d }' % spec.rowcol_graded)
+    if spec.rowcol_wave:
+        L.append('  argi { key: "rowcol-wave" val: %d }' % spec.rowcol_wave)
     L.append("}")
     if spec.conn_arrays is None:
         L.append("proc_param {")

@@ -77,6 +77,18 @@ inline int launch_status() {
   return ret(e);
 }
 
+/* grid of a looping batch launch over n items.  One workgroup per item:
+ * capping the grid at one dispatch round (each workgroup then loops over
+ * several items) was measured slower on the 512 networks (population and
+ * row/column launches 1.5x longer), so the loops run once. */
+template <typename K>
+unsigned loop_grid(K kernel, int threads, size_t lds, int64_t n) {
+  (void)kernel;
+  (void)threads;
+  (void)lds;
+  return (unsigned)n;
+}
+
 /* exclusive prefix sum across the workgroup (blockDim multiple of 64);
  * sh needs blockDim/64 + 1 ints */
 __device__ int block_excl_scan(int v, int* sh, int* total) {
@@ -816,12 +828,17 @@ __device__ __forceinline__ void apply_step(gsbn_pop_args& a, const gsbn_step_arg
   a.wmask += st.wmask_offset;
 }
 
+/* Workgroup b updates hypercolumns b, b + gridDim.x, ... of the map (the
+ * launch gives one hypercolumn to each workgroup, see loop_grid). */
 __global__ void __launch_bounds__(POP_MAX_THREADS)
     pop_kernel_batch(const gsbn_pop_args* __restrict__ arr, const int32_t* __restrict__ wg_pop,
-                     const int32_t* __restrict__ wg_hcu, gsbn_step_args st) {
-  gsbn_pop_args a = arr[wg_pop[blockIdx.x]];
-  apply_step(a, st);
-  pop_body<false>(a, wg_hcu[blockIdx.x]);
+                     const int32_t* __restrict__ wg_hcu, int nwg, gsbn_step_args st) {
+  for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads(); /* LDS of the previous hypercolumn */
+    gsbn_pop_args a = arr[wg_pop[w]];
+    apply_step(a, st);
+    pop_body<false>(a, wg_hcu[w]);
+  }
 }
 
 __global__ void pop_dsup0_kernel_batch(const gsbn_pop_args* __restrict__ arr, int npop,
@@ -854,7 +871,7 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_arg
 /* Part `part` of the row/column update of hypercolumn h.  FUSED: the
  * hypercolumn's active rows, spikes and pj come from the population phase
  * of the same workgroup (fl), not from memory. */
-template <int PREC, bool FUSED = false, bool ROWS = true>
+template <int PREC, bool FUSED = false, bool ROWS = true, int NT = RC_THREADS>
 __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcStep& q,
                                             const int h, const int part,
                                             const FusedLds* fl = nullptr) {
@@ -869,7 +886,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
    * counts, this thread's first row and the first spiking unit are read in
    * one batch, before either is known to be needed (all inside their
    * arrays). */
-  const int rpt = M >= RC_THREADS ? 1 : RC_THREADS / M;
+  const int rpt = M >= NT ? 1 : NT / M;
   int nrows, nspk, k_pre = 0, j_pre = 0;
   if (FUSED) {
     nrows = fl->nrows;
@@ -890,7 +907,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
      * read of this pass is done */
     gsbn_row_rec st0;
     int64_t r0 = -1;
-    for (int f = tid; f < nr * M; f += RC_THREADS) {
+    for (int f = tid; f < nr * M; f += NT) {
       const int rr = f / M;
       const int j = f - rr * M;
       const int aa = part + P * (base + rr);
@@ -952,14 +969,14 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   constexpr int U = RC_COLS;
   for (int s_i = 0; s_i < nspk; ++s_i) {
     const int j = FUSED ? fl->spk[s_i] : s_i == 0 ? j_pre : a.spk_list[h * M + s_i];
-    for (int kb = k0 + tid; kb < k1; kb += U * RC_THREADS) {
+    for (int kb = k0 + tid; kb < k1; kb += U * NT) {
       bool ok[U];
       int64_t sidx[U];
       int iiv[U], stamp[U];
       SynVals v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int k = kb + u * RC_THREADS;
+        const int k = kb + u * NT;
         const int r = h * C + (k < k1 ? k : k0);
         sidx[u] = (int64_t)r * M + j;
         iiv[u] = a.ii[r];
@@ -1015,15 +1032,37 @@ __global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC)) rowcol_kernel(gsbn
   rowcol_body<PREC>(a, q, h, blockIdx.x - h * a.parts);
 }
 
+/* work item i = (map entry i / parts, part i % parts); workgroup b takes
+ * items b, b + gridDim.x, ... (see loop_grid) */
 template <int PREC>
 __global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC))
     rowcol_kernel_batch(const gsbn_rowcol_args* __restrict__ arr, const int32_t* __restrict__ wg_proj,
-                        const int32_t* __restrict__ wg_hcu, int parts, gsbn_step_args st) {
-  const int w = blockIdx.x / parts;
-  const gsbn_rowcol_args& a = arr[wg_proj[w]];
-  const RcStep q = {st.simstep, st.exec, parts, a.table_lens[st.kp_slot], a.taupdt * st.prn,
-                    a.tables[st.kp_slot]};
-  rowcol_body<PREC>(a, q, wg_hcu[w], blockIdx.x - w * parts);
+                        const int32_t* __restrict__ wg_hcu, int parts, int nitems,
+                        gsbn_step_args st) {
+  for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+    const int w = i / parts;
+    const gsbn_rowcol_args& a = arr[wg_proj[w]];
+    const RcStep q = {st.simstep, st.exec, parts, a.table_lens[st.kp_slot], a.taupdt * st.prn,
+                      a.tables[st.kp_slot]};
+    rowcol_body<PREC>(a, q, wg_hcu[w], i - w * parts);
+  }
+}
+
+/* The same with one wave per hypercolumn (64-thread workgroups): a batch's
+ * hypercolumns have few active rows and rarely a spiking unit, so most of a
+ * 256-thread workgroup would idle while it holds its place on the CU; one
+ * wave each keeps four times as many hypercolumns in flight. */
+template <int PREC>
+__global__ void __launch_bounds__(64)
+    rowcol_kernel_batch_wave(const gsbn_rowcol_args* __restrict__ arr,
+                             const int32_t* __restrict__ wg_proj,
+                             const int32_t* __restrict__ wg_hcu, int nhcu, gsbn_step_args st) {
+  for (int w = blockIdx.x; w < nhcu; w += gridDim.x) {
+    const gsbn_rowcol_args& a = arr[wg_proj[w]];
+    const RcStep q = {st.simstep, st.exec, 1, a.table_lens[st.kp_slot], a.taupdt * st.prn,
+                      a.tables[st.kp_slot]};
+    rowcol_body<PREC, false, true, 64>(a, q, wg_hcu[w], 0);
+  }
 }
 
 /* The same with a work split per hypercolumn: workgroup w runs part
@@ -1033,13 +1072,14 @@ template <int PREC>
 __global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC))
     rowcol_kernel_graded(const gsbn_rowcol_args* __restrict__ arr,
                          const int32_t* __restrict__ wg_proj, const int32_t* __restrict__ wg_hcu,
-                         const int32_t* __restrict__ wg_part, gsbn_step_args st) {
-  const int w = blockIdx.x;
-  const gsbn_rowcol_args& a = arr[wg_proj[w]];
-  const int pp = wg_part[w];
-  const RcStep q = {st.simstep, st.exec, pp >> 16, a.table_lens[st.kp_slot], a.taupdt * st.prn,
-                    a.tables[st.kp_slot]};
-  rowcol_body<PREC>(a, q, wg_hcu[w], pp & 0xffff);
+                         const int32_t* __restrict__ wg_part, int nwg, gsbn_step_args st) {
+  for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
+    const gsbn_rowcol_args& a = arr[wg_proj[w]];
+    const int pp = wg_part[w];
+    const RcStep q = {st.simstep, st.exec, pp >> 16, a.table_lens[st.kp_slot], a.taupdt * st.prn,
+                      a.tables[st.kp_slot]};
+    rowcol_body<PREC>(a, q, wg_hcu[w], pp & 0xffff);
+  }
 }
 
 /* One launch per step: workgroup h updates hypercolumn h's population and
@@ -1559,8 +1599,8 @@ int gsbn_pop_update_batch(const gsbn_pop_args* pops_dev, int32_t npop, const int
     int rc = launch_status();
     if (rc) return rc;
   }
-  launch_timed(pop_kernel_batch, dim3(nwg), dim3(threads), lds, s, pops_dev, wg_pop, wg_hcu,
-                     *step);
+  launch_timed(pop_kernel_batch, dim3(loop_grid(pop_kernel_batch, threads, lds, nwg)),
+               dim3(threads), lds, s, pops_dev, wg_pop, wg_hcu, nwg, *step);
   return launch_status();
 }
 
@@ -1571,28 +1611,60 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
       step->kp_slot < 0 || step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 ||
       step->prec > GSBN_PREC_F16)
     return GSBN_ERR_ARG;
-  const int64_t grid = (int64_t)nhcu * parts;
-  if (grid > 0x7fffffff) return GSBN_ERR_UNSUPPORTED;
+  const int64_t items = (int64_t)nhcu * parts;
+  if (items > 0x7fffffff) return GSBN_ERR_UNSUPPORTED;
   hipStream_t s = (hipStream_t)stream;
-  const dim3 g((unsigned)grid), b(RC_THREADS);
+  const dim3 b(RC_THREADS);
+  const int n = (int)items;
   switch (step->prec) {
     case GSBN_PREC_F32:
-      launch_timed(rowcol_kernel_batch<GSBN_PREC_F32>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
-                         parts, *step);
+      launch_timed(rowcol_kernel_batch<GSBN_PREC_F32>,
+                   dim3(loop_grid(rowcol_kernel_batch<GSBN_PREC_F32>, RC_THREADS, 0, n)), b, 0, s,
+                   projs_dev, wg_proj, wg_hcu, parts, n, *step);
       break;
     case GSBN_PREC_BF16:
-      launch_timed(rowcol_kernel_batch<GSBN_PREC_BF16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
-                         parts, *step);
+      launch_timed(rowcol_kernel_batch<GSBN_PREC_BF16>,
+                   dim3(loop_grid(rowcol_kernel_batch<GSBN_PREC_BF16>, RC_THREADS, 0, n)), b, 0,
+                   s, projs_dev, wg_proj, wg_hcu, parts, n, *step);
       break;
     default:
-      launch_timed(rowcol_kernel_batch<GSBN_PREC_F16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
-                         parts, *step);
+      launch_timed(rowcol_kernel_batch<GSBN_PREC_F16>,
+                   dim3(loop_grid(rowcol_kernel_batch<GSBN_PREC_F16>, RC_THREADS, 0, n)), b, 0,
+                   s, projs_dev, wg_proj, wg_hcu, parts, n, *step);
       break;
   }
   return launch_status();
 }
 
 
+
+int gsbn_proj_rowcol_batch_wave(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
+                                const int32_t* wg_hcu, int32_t nhcu, const gsbn_step_args* step,
+                                void* stream) {
+  if (!projs_dev || !wg_proj || !wg_hcu || !step || nhcu <= 0 || step->kp_slot < 0 ||
+      step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 || step->prec > GSBN_PREC_F16)
+    return GSBN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 b(64);
+  switch (step->prec) {
+    case GSBN_PREC_F32:
+      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_F32>,
+                   dim3(loop_grid(rowcol_kernel_batch_wave<GSBN_PREC_F32>, 64, 0, nhcu)), b, 0, s,
+                   projs_dev, wg_proj, wg_hcu, nhcu, *step);
+      break;
+    case GSBN_PREC_BF16:
+      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_BF16>,
+                   dim3(loop_grid(rowcol_kernel_batch_wave<GSBN_PREC_BF16>, 64, 0, nhcu)), b, 0,
+                   s, projs_dev, wg_proj, wg_hcu, nhcu, *step);
+      break;
+    default:
+      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_F16>,
+                   dim3(loop_grid(rowcol_kernel_batch_wave<GSBN_PREC_F16>, 64, 0, nhcu)), b, 0,
+                   s, projs_dev, wg_proj, wg_hcu, nhcu, *step);
+      break;
+  }
+  return launch_status();
+}
 
 int gsbn_proj_rowcol_graded(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
                             const int32_t* wg_hcu, const int32_t* wg_part, int32_t nwg,
@@ -1601,19 +1673,22 @@ int gsbn_proj_rowcol_graded(const gsbn_rowcol_args* projs_dev, const int32_t* wg
       step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 || step->prec > GSBN_PREC_F16)
     return GSBN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  const dim3 g((unsigned)nwg), b(RC_THREADS);
+  const dim3 b(RC_THREADS);
   switch (step->prec) {
     case GSBN_PREC_F32:
-      launch_timed(rowcol_kernel_graded<GSBN_PREC_F32>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
-                   wg_part, *step);
+      launch_timed(rowcol_kernel_graded<GSBN_PREC_F32>,
+                   dim3(loop_grid(rowcol_kernel_graded<GSBN_PREC_F32>, RC_THREADS, 0, nwg)), b, 0,
+                   s, projs_dev, wg_proj, wg_hcu, wg_part, nwg, *step);
       break;
     case GSBN_PREC_BF16:
-      launch_timed(rowcol_kernel_graded<GSBN_PREC_BF16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
-                   wg_part, *step);
+      launch_timed(rowcol_kernel_graded<GSBN_PREC_BF16>,
+                   dim3(loop_grid(rowcol_kernel_graded<GSBN_PREC_BF16>, RC_THREADS, 0, nwg)), b,
+                   0, s, projs_dev, wg_proj, wg_hcu, wg_part, nwg, *step);
       break;
     default:
-      launch_timed(rowcol_kernel_graded<GSBN_PREC_F16>, g, b, 0, s, projs_dev, wg_proj, wg_hcu,
-                   wg_part, *step);
+      launch_timed(rowcol_kernel_graded<GSBN_PREC_F16>,
+                   dim3(loop_grid(rowcol_kernel_graded<GSBN_PREC_F16>, RC_THREADS, 0, nwg)), b,
+                   0, s, projs_dev, wg_proj, wg_hcu, wg_part, nwg, *step);
       break;
   }
   return launch_status();

@@ -1180,6 +1180,9 @@ class ProcUpdLazy : public ProcedureBase {
    * activity per candidate and chose worse plans.) */
   static constexpr int kTuneStart = 20, kTuneRun = 1, kTuneSamples = 10;
   int _parts_fixed = 0, _fused_fixed = 0;
+  /* rowcol-wave 1: a batch's row/column update with one wave per
+   * hypercolumn (plan -20) */
+  int _wave_fixed = 0;
   bool _tuned = false;
   int _tune_pos = 0, _run_len = 0;
   vector<int> _cand;
@@ -1219,6 +1222,7 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
   if (_fused_fixed) return -1;
   if (_chain_fixed > 0) return -2;
   if (_graded_fixed > 0) return -10 - _graded_fixed;
+  if (_wave_fixed > 0) return -20;
   if (_parts_fixed > 0) return _parts_fixed;
   if (_tuned || execs < kTuneStart) return _parts;
   if (_cand.empty()) {
@@ -1236,6 +1240,7 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
     if (_cand.empty()) _cand.push_back(1);
     if (_fusable) _cand.push_back(-1);
     if (_chainable) _cand.push_back(-2);
+    if (_batch && !_hcu_work.empty()) _cand.push_back(-20);
     /* graded splits; their maps are built here, before any sample is timed */
     if (_batch && !_hcu_work.empty())
       for (int g = 1; g <= 3; ++g) {
@@ -1301,6 +1306,7 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   par.argi("chained-step", _chain_fixed);
   /* rowcol-graded g (1-3): always the graded split g of a batch */
   par.argi("rowcol-graded", _graded_fixed);
+  par.argi("rowcol-wave", _wave_fixed);
   if (_graded_fixed < 0 || _graded_fixed > 3)
     throw Error(-2, "ProcUpdLazy: rowcol-graded must be in [0, 3]");
   if (_chain_fixed && _fused_fixed > 0 && _chain_fixed > 0)
@@ -1541,14 +1547,17 @@ void ProcUpdLazy::update_gpu() {
     }
     if (_graded_fixed && !_batch)
       throw Error(-2, "ProcUpdLazy: rowcol-graded needs a batch of populations or projections");
+    if (_wave_fixed > 0 && !_batch)
+      throw Error(-2, "ProcUpdLazy: rowcol-wave needs a batch of populations or projections");
     _parts = _fused_fixed ? -1 : (_chain_fixed > 0 ? -2 : _parts0);
     _glv.puti("rowcol-parts", _fused_fixed ? -1
                               : _chain_fixed > 0 ? -2
                               : _graded_fixed > 0 ? -10 - _graded_fixed
+                              : _wave_fixed > 0 ? -20
                               : _parts_fixed > 0 ? _parts_fixed : _parts);
     /* GlobalVar plan-tuned: 1 once the plan no longer changes */
     _glv.puti("plan-tuned", (_fused_fixed || _chain_fixed > 0 || _graded_fixed > 0 ||
-                             _parts_fixed > 0 || _list_proj.empty()) ? 1 : 0);
+                             _wave_fixed > 0 || _parts_fixed > 0 || _list_proj.empty()) ? 1 : 0);
     _configured = true;
     _args_dirty = true;
   }
@@ -1689,7 +1698,11 @@ void ProcUpdLazy::update_gpu() {
     }
     if (!_list_proj.empty()) {
       Timing::get().begin(1, stream);
-      if (_batch && plan <= -10) {
+      if (_batch && plan == -20) {
+        op_check(gsbn_proj_rowcol_batch_wave(_proj_dev.p, _wg_proj.p, _wg_proj_hcu.p, _nhcu_proj,
+                                             &st, stream),
+                 "gsbn_proj_rowcol_batch_wave");
+      } else if (_batch && plan <= -10) {
         const int g = -10 - plan;
         if (!_gmap_n[g]) build_graded_map(g);
         op_check(gsbn_proj_rowcol_graded(_proj_dev.p, _gmap_proj[g].p, _gmap_hcu[g].p,

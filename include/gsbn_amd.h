@@ -253,6 +253,13 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
                            const int32_t* wg_hcu, int32_t nhcu, int32_t parts,
                            const gsbn_step_args* step, void* stream);
 
+/* gsbn_proj_rowcol_batch with one 64-thread workgroup (one wave) per
+ * hypercolumn of the map: same results, more hypercolumns in flight when
+ * each has little work. */
+int gsbn_proj_rowcol_batch_wave(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
+                                const int32_t* wg_hcu, int32_t nhcu, const gsbn_step_args* step,
+                                void* stream);
+
 /* Batched row/column update with a work split per hypercolumn: workgroup w
  * runs part (wg_part[w] & 0xffff) of (wg_part[w] >> 16) parts of hypercolumn
  * wg_hcu[w] of projection wg_proj[w]; every part of a hypercolumn must be in

@@ -2,6 +2,10 @@ import os
 import sys
 
 import pytest
+# torch first: it brings its own HIP runtime, and libgsbn_amd.so then binds to
+# that one (same soname).  Loaded the other way round, the process would hold
+# two HIP runtimes, and device memory of one is not usable through the other.
+import torch  # noqa: F401
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "oracle")):

@@ -261,3 +261,15 @@ def test_batch512_sampled_networks_bitwise(tmp_path):
             assert not bad, (t, bad[:4])
     gpu.close()
     ora.close()
+
+
+def test_batched_networks_wave_plan(tmp_path):
+    # the batch's row/column update with one wave per hypercolumn (plan -20)
+    spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=(3, 140), n_pat=4, train=40,
+                                quiet=5, recall=30, reset=20)
+    spec.rowcol_wave = 1
+    gpu, ora = _lockstep_and_compare(spec, tmp_path)
+    c, t, flags = gpu.check()
+    assert (c, t) == (ora.check.correct, ora.check.total)
+    assert gpu.glob("rowcol-parts") == -20
+    gpu.close()
