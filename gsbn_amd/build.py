@@ -25,7 +25,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__",
               "-I" + os.path.join(ROCM, "include"), "-Wall", "-Wno-unused-function"] + FP_FLAGS
 
-HIP_SOURCES = ["kernels.hip", "recall.hip"]
+HIP_SOURCES = ["kernels.hip", "recall.hip", "noalloc.hip"]
 HOST_SOURCES = ["framework.cpp", "procedures.cpp", "proc_upd_lazy.cpp", "capi.cpp", "coeff.cpp",
                 "snapshot.cpp", "structural_plasticity.cpp"]
 

@@ -164,6 +164,17 @@ typedef struct {
 } gsbn_pop_args;
 int gsbn_pop_update(const gsbn_pop_args* a, void* stream);
 
+/* gsbn_pop_update without the workspace argument: scratch_dsup0 is ignored
+ * and the workspace (one float, the population's first unit's dsup when
+ * wta_pop_max) is taken from a fixed device array of 64 floats that the
+ * library holds per GPU until the process ends.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing; every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU (on different streams) race
+ * and must be serialised by the caller. */
+int gsbn_pop_update_noalloc(const gsbn_pop_args* a, void* stream);
+
 /* Apply pending support contributions to epsc without a population update
  * (used before host reads of epsc_<pop>). Grid: one workgroup per HCU. */
 int gsbn_pop_flush(const gsbn_pop_args* a, void* stream);
