@@ -72,8 +72,13 @@ def workload(name: str):
     raise SystemExit("unknown workload " + name)
 
 
+DEFERRED = 0  # --deferred K, applied with the plan to every measured spec
+
+
 def apply_plan(spec, plan: str):
-    """--plan: auto leaves the choice to the online tuner."""
+    """--plan: auto leaves the choice to the online tuner; --deferred K
+    defers the column updates."""
+    spec.deferred_columns = DEFERRED
     if plan == "auto":
         return
     if plan == "fused":
@@ -428,6 +433,7 @@ def hbm_copy(gib: int = 2, reps: int = 10) -> dict:
 def batched(steps: int, warmup: int) -> dict:
     """The 512-network form: aggregate and per-network step rates."""
     spec, desc = workload("batch512")
+    spec.deferred_columns = DEFERRED
     m = measure(spec, steps, warmup, step_times=100)
     n = len(spec.pops)
     return {"workload": desc, "networks": n, "steps": m["done"],
@@ -513,11 +519,16 @@ def main():
     ap.add_argument("--plan", default="auto",
                     help="step plan: auto (tuned online), split:N (N row/column workgroups "
                          "per HCU), fused, chained, or wave (batches: one wave per HCU)")
+    ap.add_argument("--deferred", type=int, default=0, metavar="K",
+                    help="deferred column updates with rings of K spike steps per unit "
+                         "(ProcUpdLazy deferred-columns; 0: eager)")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--cpu-arith", default="ref32", choices=("ref32", "table"))
     ap.add_argument("--step-times", type=int, default=0,
                     help="after the timed region, time this many synchronised single steps")
     args = ap.parse_args()
+    global DEFERRED
+    DEFERRED = args.deferred
     if args.cpu_only:
         return cpu_only(args)
     if args.gpus != 1:

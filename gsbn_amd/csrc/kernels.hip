@@ -301,6 +301,14 @@ struct SynIO {
       *reinterpret_cast<uint4*>(syn16 + i) = pack16(s);
     }
   }
+  /* the values a store and a load would leave (16-bit forms round Pij and
+   * Eij to their storage precision) */
+  __device__ __forceinline__ void round_traces(SynVals& s) const {
+    if constexpr (PREC != GSBN_PREC_F32) {
+      s.pij = gsbn_unpack_trace(PREC, (uint16_t)gsbn_pack_trace(PREC, s.pij, ps), pinv);
+      s.eij = gsbn_unpack_trace(PREC, (uint16_t)gsbn_pack_trace(PREC, s.eij, es), einv);
+    }
+  }
   __device__ __forceinline__ void store_tij(int64_t i, int t) const {
     if constexpr (PREC == GSBN_PREC_F32)
       syn[i].tij = t;
@@ -308,6 +316,32 @@ struct SynIO {
       syn16[i].tij = t;
   }
 };
+
+/* Deferred column updates: the pending column kicks of one synapse.  ring:
+ * unit (h, j)'s ring of K spike steps (K a power of two), T: the unit's
+ * spike count the ring is read at.  Applies, oldest first, the steps s with
+ * v.tij < s <= upto among the last K - 1 entries, each as the reference's
+ * column update of step s (Proj.cpp:481-536): catch up over s - tij, zj2 +=
+ * kftj, tij = s, and (16-bit forms) Pij and Eij rounded as the store
+ * between two updates rounds them.  Returns whether a kick was applied. */
+template <int PREC>
+__device__ __forceinline__ bool replay_kicks(const SynIO<PREC>& io, SynVals& v,
+                                             const int32_t* ring, int T, int K, int upto,
+                                             const float* table, int table_len, float kftj) {
+  const int lo = T - K + 1 > 0 ? T - K + 1 : 0;
+  int n1 = T - 1;
+  while (n1 >= lo && ring[n1 & (K - 1)] > upto) --n1;
+  int n0 = n1 + 1;
+  while (n0 > lo && ring[(n0 - 1) & (K - 1)] > v.tij) --n0;
+  for (int n = n0; n <= n1; ++n) {
+    const int sn = ring[n & (K - 1)];
+    gsbn_syn_catchup_k(load_coef(table, table_len, sn - v.tij), &v.pij, &v.eij, &v.zi2, &v.zj2);
+    v.zj2 = v.zj2 + kftj;
+    v.tij = sn;
+    io.round_traces(v);
+  }
+  return n0 <= n1;
+}
 
 /* per-launch values of a row/column update (taken from the argument struct
  * in the single form, from the step arguments in the batched form) */
@@ -365,6 +399,15 @@ __device__ __forceinline__ float chained_rows(const gsbn_rowcol_args& b, const R
         st[u] = b.rowst[r];
         if (live) sv[u] = io.load(r * M + j, !learn);
       }
+    }
+    if (b.dK && live) {
+      /* deferred column updates: this synapse's pending kicks first */
+      const int uj = h * M + j;
+      const int32_t* ring = b.dhist + (int64_t)uj * b.dK;
+      const int T = b.dcnt[uj];
+      for (int u = 0; u < 4; ++u)
+        if (base + u < nrows)
+          replay_kicks<PREC>(io, sv[u], ring, T, b.dK, t - 1, q.table, q.table_len, b.kftj);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -755,6 +798,36 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     }
   }
 
+  /* 6b. deferred column updates (first projection): this step's spikes go
+   * into the postsynaptic units' rings; a unit whose spikes pending since
+   * its last column walk reach dK - 1 is listed for a walk of this step */
+  if (nin > 0 && a.proj0.dK) {
+    const gsbn_pop_proj& p0 = a.proj0;
+    const int K = p0.dK;
+    int wflag = 0, wT = 0;
+    if (live && spk) {
+      const int n = p0.dcnt[idx];
+      p0.dhist[(int64_t)idx * K + (n & (K - 1))] = a.simstep;
+      p0.dcnt[idx] = n + 1;
+      if (n + 1 - p0.dfl[idx] >= K - 1) {
+        p0.dfl[idx] = n + 1;
+        wflag = 1;
+        wT = n + 1;
+      }
+    }
+    if (spk_tot > 0) {
+      int wtot;
+      const int pos = block_scan_fast(wflag, sh_scan, &wtot);
+      if (wflag) {
+        p0.walk_list[h * M + pos] = j;
+        p0.walk_T[h * M + pos] = wT;
+      }
+      if (j == 0) p0.walk_cnt[h] = wtot;
+    } else if (j == 0) {
+      p0.walk_cnt[h] = 0;
+    }
+  }
+
   /* 7. active rows of this step (Proj.cpp:621-630): the ring slot of this
    * exec step, listed in ascending order and cleared for its next use */
   for (int m = 0; m < a.nproj_in; ++m) {
@@ -917,6 +990,12 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
       const int64_t sidx = r * M + j;
       gsbn_row_rec st = a.rowst[r];
       SynVals s = io.load(sidx, !learn);
+      if (a.dK) {
+        /* deferred column updates: this synapse's pending kicks first */
+        const int uj = h * M + j;
+        replay_kicks<PREC>(io, s, a.dhist + (int64_t)uj * a.dK, a.dcnt[uj], a.dK, t - 1, q.table,
+                           q.table_len, a.kftj);
+      }
       const int Dr = t - st.ti;
       if (Dr <= 0) {
         st.zi += a.kfti;
@@ -967,6 +1046,35 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
    * different slots are in flight together: slot checks, records,
    * coefficients, then the updates */
   constexpr int U = RC_COLS;
+  if (!FUSED && a.dK) {
+    /* deferred column updates: walk the listed units' columns, applying
+     * their pending kicks (up to this step's) to every slot with ii >= 0
+     * that this step's row pass does not own */
+    for (int s_i = 0; s_i < nspk; ++s_i) {
+      const int j = s_i == 0 ? j_pre : a.spk_list[h * M + s_i];
+      const int uj = h * M + j;
+      const int32_t* ring = a.dhist + (int64_t)uj * a.dK;
+      const int T = a.walk_T[h * M + s_i];
+      for (int kb = k0 + tid; kb < k1; kb += U * NT) {
+        bool ok[U];
+        int64_t sidx[U];
+        SynVals v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = kb + u * NT;
+          const int r = h * C + (k < k1 ? k : k0);
+          sidx[u] = (int64_t)r * M + j;
+          ok[u] = (k < k1) & (a.ii[r] >= 0) & (a.act_stamp[r] != q.exec);
+          v[u] = io.load_traces(sidx[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (ok[u] && replay_kicks<PREC>(io, v[u], ring, T, a.dK, t, q.table, q.table_len, a.kftj))
+            io.store_traces(sidx[u], v[u]);
+      }
+    }
+    return;
+  }
   for (int s_i = 0; s_i < nspk; ++s_i) {
     const int j = FUSED ? fl->spk[s_i] : s_i == 0 ? j_pre : a.spk_list[h * M + s_i];
     for (int kb = k0 + tid; kb < k1; kb += U * NT) {
@@ -1140,6 +1248,7 @@ __global__ void __launch_bounds__(RC_THREADS) chained_kernel(gsbn_pop_args a, gs
 template <int PREC>
 __global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
   __shared__ float sh_pi[ALL_ROWS];
+  __shared__ int sh_ii[ALL_ROWS];
   const SynIO<PREC> io(a);
   const int64_t R = (int64_t)a.H * a.C;
   const int M = a.M;
@@ -1161,6 +1270,7 @@ __global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
         a.rowst[r] = st;
       }
       sh_pi[tid] = st.pi;
+      sh_ii[tid] = a.dK ? a.ii[r] : -1;
     }
     __syncthreads();
     for (int f = tid; f < nr * M; f += ALL_THREADS) {
@@ -1169,9 +1279,22 @@ __global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
       const int64_t r = r0 + rr;
       const int64_t sidx = r * M + j;
       SynVals s = io.load(sidx, false);
+      bool kicked = false;
+      if (sh_ii[rr] >= 0) {
+        /* deferred column updates: the pending kicks up to the target step,
+         * with the old learning rate, as the eager column updates had them */
+        const int uj = (int)(r / a.C) * M + j;
+        kicked = replay_kicks<PREC>(io, s, a.dhist + (int64_t)uj * a.dK, a.dcnt[uj], a.dK, t,
+                                    a.table, a.table_len, a.kftj);
+      }
       const int D = t - s.tij;
       if (D <= 0) {
-        io.store_tij(sidx, t);
+        if (kicked) {
+          s.tij = t;
+          io.store(sidx, s, false);
+        } else {
+          io.store_tij(sidx, t);
+        }
       } else {
         gsbn_syn_catchup_k(load_coef(a.table, a.table_len, D), &s.pij, &s.eij, &s.zi2, &s.zj2);
         s.tij = t;
@@ -1183,6 +1306,30 @@ __global__ void __launch_bounds__(ALL_THREADS) all_kernel(gsbn_all_args a) {
       }
     }
     __syncthreads();
+  }
+}
+
+/* gsbn_proj_materialize: every pending column kick of every slot with
+ * ii >= 0, applied in place (grid-stride over chunks of rows, a row's M
+ * records contiguous) */
+template <int PREC>
+__global__ void __launch_bounds__(ALL_THREADS) materialize_kernel(gsbn_all_args a) {
+  const SynIO<PREC> io(a);
+  const int64_t R = (int64_t)a.H * a.C;
+  const int M = a.M;
+  const int64_t n = R * M;
+  for (int64_t f = (int64_t)blockIdx.x * ALL_THREADS + threadIdx.x; f < n;
+       f += (int64_t)gridDim.x * ALL_THREADS) {
+    const int64_t r = f / M;
+    const int j = (int)(f - r * M);
+    if (a.ii[r] < 0) continue;
+    const int uj = (int)(r / a.C) * M + j;
+    const int T = a.dcnt[uj];
+    if (T == 0) continue;
+    SynVals s = io.load_traces(f);
+    if (replay_kicks<PREC>(io, s, a.dhist + (int64_t)uj * a.dK, T, a.dK, a.target, a.table,
+                           a.table_len, a.kftj))
+      io.store_traces(f, s);
   }
 }
 
@@ -1467,6 +1614,12 @@ static bool trace_form_ok(const A& a) {
 /* dynamic LDS of the population update (pop_body): three arrays of
  * `threads` entries and the active-row bitmap; the fused step adds the
  * active-row list */
+/* deferred column updates: dK 0 (off) or a power of two in [2, 64] with its
+ * arrays */
+static bool deferred_ok(int32_t dK, const void* ii, const void* dhist, const void* dcnt) {
+  return dK == 0 || (dK >= 2 && dK <= 64 && (dK & (dK - 1)) == 0 && ii && dhist && dcnt);
+}
+
 static size_t pop_lds(int threads, int64_t max_slots, bool fused) {
   return (size_t)(3 * (int64_t)threads + (max_slots + 31) / 32 + (fused ? max_slots : 0)) *
          sizeof(uint32_t);
@@ -1480,6 +1633,10 @@ int gsbn_pop_update(const gsbn_pop_args* a, void* stream) {
       a->nproj_out < 0 || (a->nproj_out > 0 && !a->outs))
     return GSBN_ERR_ARG;
   if (a->wta_pop_max && !a->scratch_dsup0) return GSBN_ERR_ARG;
+  if (a->nproj_in > 0 && a->proj0.dK &&
+      (!deferred_ok(a->proj0.dK, a->proj0.rows_list, a->proj0.dhist, a->proj0.dcnt) ||
+       !a->proj0.dfl || !a->proj0.walk_list || !a->proj0.walk_cnt || !a->proj0.walk_T))
+    return GSBN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   const int threads = ((a->M + 63) / 64) * 64;
   const size_t lds = pop_lds(threads, a->max_slots, false);
@@ -1502,9 +1659,31 @@ int gsbn_pop_flush(const gsbn_pop_args* a, void* stream) {
   return launch_status();
 }
 
+int gsbn_proj_materialize(const gsbn_all_args* a, void* stream) {
+  if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2 ||
+      !trace_form_ok(*a) || a->dK <= 0 || !deferred_ok(a->dK, a->ii, a->dhist, a->dcnt))
+    return GSBN_ERR_ARG;
+  const int64_t n = (int64_t)a->H * a->C * a->M;
+  if (n == 0) return 0;
+  const int grid = grid_for(n, ALL_THREADS, 16384);
+  hipStream_t s = (hipStream_t)stream;
+  switch (a->prec) {
+    case GSBN_PREC_F32:
+      hipLaunchKernelGGL(materialize_kernel<GSBN_PREC_F32>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      break;
+    case GSBN_PREC_BF16:
+      hipLaunchKernelGGL(materialize_kernel<GSBN_PREC_BF16>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      break;
+    default:
+      hipLaunchKernelGGL(materialize_kernel<GSBN_PREC_F16>, dim3(grid), dim3(ALL_THREADS), 0, s, *a);
+      break;
+  }
+  return launch_status();
+}
+
 int gsbn_proj_all(const gsbn_all_args* a, void* stream) {
   if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2 ||
-      !trace_form_ok(*a))
+      !trace_form_ok(*a) || !deferred_ok(a->dK, a->ii, a->dhist, a->dcnt))
     return GSBN_ERR_ARG;
   const int64_t R = (int64_t)a->H * a->C;
   if (R == 0) return 0;
@@ -1557,7 +1736,8 @@ int gsbn_rowcol_resident(int32_t prec, int32_t* out) {
 
 int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream) {
   if (!a || a->H <= 0 || a->M <= 0 || a->C < 0 || !a->table || a->table_len < 2 || a->parts < 0 ||
-      !trace_form_ok(*a))
+      !trace_form_ok(*a) || !deferred_ok(a->dK, a->ii, a->dhist, a->dcnt) ||
+      (a->dK && !a->walk_T))
     return GSBN_ERR_ARG;
   gsbn_rowcol_args b = *a;
   if (b.parts == 0) {
@@ -1701,6 +1881,8 @@ int gsbn_step_fused(const gsbn_pop_args* a, const gsbn_rowcol_args* b, void* str
       !trace_form_ok(*b))
     return GSBN_ERR_ARG;
   if (a->wta_pop_max && !a->scratch_dsup0) return GSBN_ERR_ARG;
+  /* the fused row/column phase does eager column updates only */
+  if (b->dK || a->proj0.dK) return GSBN_ERR_UNSUPPORTED;
   const size_t lds = pop_lds(RC_THREADS, a->max_slots, true);
   if (lds > 48 * 1024) return GSBN_ERR_UNSUPPORTED;
   hipStream_t s = (hipStream_t)stream;

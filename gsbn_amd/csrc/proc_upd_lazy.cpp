@@ -218,6 +218,41 @@ class Proj : public ProjRowOps {
   void flush_deferred() {
     if (flush_hook) flush_hook();
   }
+  /* Deferred column updates (dK > 0, see gsbn_pop_proj): each
+   * postsynaptic unit's recent spike steps in a ring, the column update of
+   * a spike replayed by the row update that next touches a synapse, by a
+   * column walk once dK - 1 spikes of a unit are pending, by a sweep, or by
+   * materialize_kicks() before the synapse state is read or changed from the
+   * host.  Two walk-list sets alternate by step in the chained plan. */
+  int dK = 0;
+  DevBuf<int32_t> dhist, dcnt, dfl, walk_list[2], walk_cnt[2], walk_T[2];
+  /* the device copy of ii_, as the kernels saw it (not synchronised from a
+   * newer host copy: pending kicks belong to the connectivity they arose in) */
+  const int32_t* ii_dev = nullptr;
+  int kick_t = 0;        /* the last executed step (kicks up to it may be pending) */
+  float kick_kp = 0.0f;  /* its learning rate */
+  bool kicks_pending = false;
+  void enable_deferred(int K) {
+    const size_t N = (size_t)_dim_hcu * _dim_mcu;
+    dK = K;
+    ii_dev = _ii->gpu_data();
+    dhist.alloc(N * (size_t)K);
+    dcnt.alloc(N);
+    dfl.alloc(N);
+    op_check(gsbn_i32_fill(dcnt.p, (int64_t)N, 0, active_stream()), "fill");
+    op_check(gsbn_i32_fill(dfl.p, (int64_t)N, 0, active_stream()), "fill");
+    op_check(gsbn_i32_fill(dhist.p, (int64_t)(N * (size_t)K), 0, active_stream()), "fill");
+    for (int i = 0; i < 2; ++i) {
+      walk_list[i].alloc(N);
+      walk_T[i].alloc(N);
+      walk_cnt[i].alloc((size_t)_dim_hcu);
+      op_check(gsbn_i32_fill(walk_cnt[i].p, _dim_hcu, 0, active_stream()), "fill");
+    }
+  }
+  void materialize_kicks();
+  /* deferral ends (the kicks pending are applied first): structural changes
+   * of the rows are not replayed; set by ProcUpdLazy */
+  std::function<void()> eager_hook;
   DevBuf<uint8_t> sp_flags; /* structural plasticity scratch */
   DevBuf<int32_t> sp_rows;
   /* delay queue: rows of each source unit (CSR) with their delays, and the
@@ -268,6 +303,7 @@ struct SynField : VecView {
   SynField(Proj* q, int f) : p(q), field(f) {}
   void pull(VecBase* v) override {
     p->flush_deferred();
+    p->materialize_kicks();
     const size_t n = (size_t)p->nsyn;
     DevBuf<uint32_t> tmp;
     tmp.alloc(n);
@@ -826,6 +862,15 @@ gsbn_pop_proj Proj::pop_desc() {
   d.rows_list = rows_list.p;
   d.rows_cnt = rows_cnt.p;
   d.act_stamp = act_stamp.p;
+  if (dK) {
+    d.dK = dK;
+    d.dhist = dhist.p;
+    d.dcnt = dcnt.p;
+    d.dfl = dfl.p;
+    d.walk_list = walk_list[0].p;
+    d.walk_cnt = walk_cnt[0].p;
+    d.walk_T = walk_T[0].p;
+  }
   return d;
 }
 
@@ -901,9 +946,45 @@ void Proj::update_all(int simstep, float old_prn) {
   a.wij = wij16.p;
   a.p_scale = p_scale();
   a.e_scale = e_scale();
+  if (dK) {
+    a.dK = dK;
+    a.ii = _ii->gpu_data();
+    a.dhist = dhist.p;
+    a.dcnt = dcnt.p;
+    a.kftj = _kftj;
+  }
   Timing::get().begin(2, active_stream());
   op_check(gsbn_proj_all(&a, active_stream()), "gsbn_proj_all");
   Timing::get().end(2, active_stream());
+}
+
+/* every pending column kick, applied (the state the reference's eager
+ * column updates leave); the learning rate has not changed since the kicks
+ * (a change sweeps them up) */
+void Proj::materialize_kicks() {
+  if (!dK || !kicks_pending) return;
+  kicks_pending = false;
+  const CoefTable& t = table(kick_kp, (int64_t)std::max(kick_t, 0) + 2);
+  gsbn_all_args a;
+  memset(&a, 0, sizeof(a));
+  a.H = _dim_hcu;
+  a.M = _dim_mcu;
+  a.C = _dim_conn;
+  a.target = kick_t;
+  a.table = t.dev.p;
+  a.table_len = (int32_t)t.len;
+  a.syn = syn.p;
+  a.prec = prec;
+  a.syn16 = syn16.p;
+  a.wij = wij16.p;
+  a.p_scale = p_scale();
+  a.e_scale = e_scale();
+  a.dK = dK;
+  a.ii = ii_dev;
+  a.dhist = dhist.p;
+  a.dcnt = dcnt.p;
+  a.kftj = _kftj;
+  op_check(gsbn_proj_materialize(&a, active_stream()), "gsbn_proj_materialize");
 }
 
 void Proj::update_jxx(float prn) {
@@ -961,6 +1042,15 @@ gsbn_rowcol_args Proj::static_args(const float* slot_prn, int nslots, int64_t ne
   a.wij = wij16.p;
   a.p_scale = p_scale();
   a.e_scale = e_scale();
+  if (dK) {
+    /* the column pass walks the listed units instead of every spiking one */
+    a.dK = dK;
+    a.dhist = dhist.p;
+    a.dcnt = dcnt.p;
+    a.spk_list = walk_list[0].p;
+    a.spk_cnt = walk_cnt[0].p;
+    a.walk_T = walk_T[0].p;
+  }
   for (int i = 0; i < nslots; ++i) {
     const CoefTable& t = table(_taupdt * slot_prn[i], need);
     a.tables[i] = t.dev.p;
@@ -1031,6 +1121,7 @@ Proj::~Proj() {
 
 void Proj::prune_flags(int t_min, float wp, float wn, vector<uint8_t>* flags) {
   flush_deferred();
+  if (eager_hook) eager_hook();
   const int64_t R = (int64_t)_dim_hcu * _dim_conn;
   prepare(0); /* host writes of ii and the traces reach the device first */
   gsbn_prune_args a;
@@ -1054,6 +1145,7 @@ void Proj::prune_flags(int t_min, float wp, float wn, vector<uint8_t>* flags) {
 
 void Proj::clear_rows(const vector<int32_t>& rows) {
   flush_deferred();
+  if (eager_hook) eager_hook();
   if (rows.empty()) return;
   const int64_t R = (int64_t)_dim_hcu * _dim_conn;
   for (int32_t r : rows)
@@ -1080,6 +1172,7 @@ void Proj::clear_rows(const vector<int32_t>& rows) {
 
 void Proj::rows_added() {
   flush_deferred();
+  if (eager_hook) eager_hook();
   /* the fan-out lists follow ii/di now */
   prepare(0);
   /* with the population kernel doing the queue push, the spikes of the last
@@ -1183,6 +1276,10 @@ class ProcUpdLazy : public ProcedureBase {
   /* rowcol-wave 1: a batch's row/column update with one wave per
    * hypercolumn (plan -20) */
   int _wave_fixed = 0;
+  /* deferred-columns K: column updates deferred through rings of K spike
+   * steps per unit (0: eager); see Proj::dK */
+  int _defer_k = 0;
+  void end_deferral();
   bool _tuned = false;
   int _tune_pos = 0, _run_len = 0;
   vector<int> _cand;
@@ -1213,6 +1310,21 @@ void ProcUpdLazy::flush_deferred() {
   Timing::get().begin(1, s);
   op_check(gsbn_proj_rowcol(&r, s), "gsbn_proj_rowcol");
   Timing::get().end(1, s);
+}
+
+/* column updates become eager again (a structural change of the rows is
+ * coming, which the replay of pending kicks cannot follow): the pending
+ * row/column work and every pending kick are applied first */
+void ProcUpdLazy::end_deferral() {
+  if (!_defer_k) return;
+  flush_deferred();
+  for (auto* q : _list_proj) {
+    q->materialize_kicks();
+    q->dK = 0;
+  }
+  _defer_k = 0;
+  _args_dirty = true;
+  _glv.puti("deferred-columns", 0);
 }
 
 /* the plan of this step; *timed while tuning (the step belongs to run
@@ -1307,6 +1419,9 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   /* rowcol-graded g (1-3): always the graded split g of a batch */
   par.argi("rowcol-graded", _graded_fixed);
   par.argi("rowcol-wave", _wave_fixed);
+  par.argi("deferred-columns", _defer_k);
+  if (_defer_k < 0 || _defer_k == 1 || _defer_k > 64 || (_defer_k & (_defer_k - 1)))
+    throw Error(-2, "ProcUpdLazy: deferred-columns must be 0 or a power of two in [2, 64]");
   if (_graded_fixed < 0 || _graded_fixed > 3)
     throw Error(-2, "ProcUpdLazy: rowcol-graded must be in [0, 3]");
   if (_chain_fixed && _fused_fixed > 0 && _chain_fixed > 0)
@@ -1545,6 +1660,25 @@ void ProcUpdLazy::update_gpu() {
       _chain_parts = std::max(1, std::min(64, std::max(fit, std::min(by_slots, budget))));
       q->flush_hook = [this]() { flush_deferred(); };
     }
+    /* deferred column updates: every population with exactly one incoming
+     * projection (the population kernel keeps the first projection's rings) */
+    if (_defer_k) {
+      bool ok = !_list_proj.empty();
+      for (auto* p : _list_pop)
+        if (p->_in.size() != 1) ok = false;
+      if (!ok) _defer_k = 0;
+    }
+    if (_defer_k) {
+      if (_fused_fixed)
+        throw Error(-2, "ProcUpdLazy: fused-step runs eager column updates only "
+                        "(deferred-columns 0)");
+      _fusable = false;
+      for (auto* q : _list_proj) {
+        q->enable_deferred(_defer_k);
+        q->eager_hook = [this]() { end_deferral(); };
+      }
+    }
+    _glv.puti("deferred-columns", _defer_k);
     if (_graded_fixed && !_batch)
       throw Error(-2, "ProcUpdLazy: rowcol-graded needs a batch of populations or projections");
     if (_wave_fixed > 0 && !_batch)
@@ -1567,7 +1701,10 @@ void ProcUpdLazy::update_gpu() {
   /* host writes reach the device below: pending work of a chained step
    * runs first, on the state it was computed from (reads need no flush:
    * the views of the synapse and row state flush when they are read) */
-  if (_pending && VecBase::host_write_epoch() != _write_epoch_seen) flush_deferred();
+  if (VecBase::host_write_epoch() != _write_epoch_seen) {
+    if (_pending) flush_deferred();
+    for (auto* q : _list_proj) q->materialize_kicks();
+  }
   _write_epoch_seen = VecBase::host_write_epoch();
   if (touched)
     for (auto* q : _list_proj) q->prepare(simstep);
@@ -1639,6 +1776,11 @@ void ProcUpdLazy::update_gpu() {
     pa.spk_list = dp->spk_list_at(side);
     pa.spk_cnt = dp->spk_cnt_at(side);
     pa.proj0.act_stamp = q->act_stamp_at(side);
+    if (q->dK) {
+      pa.proj0.walk_list = q->walk_list[side].p;
+      pa.proj0.walk_cnt = q->walk_cnt[side].p;
+      pa.proj0.walk_T = q->walk_T[side].p;
+    }
     if (_pending) {
       const CoefTable& t = q->table(_pend_ra.kp, (int64_t)_pend_ra.simstep + 2);
       _pend_ra.table = t.dev.p;
@@ -1653,6 +1795,11 @@ void ProcUpdLazy::update_gpu() {
     dp->spk_side = side;
     ra.spk_list = pa.spk_list;
     ra.spk_cnt = pa.spk_cnt;
+    if (q->dK) {
+      ra.spk_list = q->walk_list[side].p;
+      ra.spk_cnt = q->walk_cnt[side].p;
+      ra.walk_T = q->walk_T[side].p;
+    }
     ra.act_stamp = pa.proj0.act_stamp;
     ra.parts = _chain_parts;
     _pend_ra = ra;
@@ -1718,6 +1865,12 @@ void ProcUpdLazy::update_gpu() {
     }
   }
   for (auto* q : _list_proj) q->update_que(_exec);
+  for (auto* q : _list_proj)
+    if (q->dK) {
+      q->kick_t = simstep;
+      q->kick_kp = q->_taupdt * prn;
+      q->kicks_pending = true;
+    }
   if (timed && ++_run_len == kTuneRun) {
     hip_check(hipEventRecord(_tev[ev + 1], stream), "hipEventRecord");
     _run_len = 0;

@@ -100,6 +100,21 @@ typedef struct {
   int32_t* rows_list;
   int32_t* rows_cnt;
   int32_t* act_stamp;
+  /* Deferred column updates (dK > 0, a power of two): the population
+   * kernel records each spike of postsynaptic unit u at step s in u's ring,
+   * dhist[u*dK + n % dK] = s for its n-th spike (dcnt[u] spikes so far), and
+   * the column update of that spike waits: rows replay it when they are
+   * next updated (see gsbn_rowcol_args).  When a unit's pending spikes since
+   * its last column walk (dfl[u] = dcnt[u] then) reach dK - 1, the unit is
+   * listed for a walk this step: walk_list/walk_cnt per HCU (ascending), with
+   * dcnt[u] at listing in walk_T. */
+  int32_t* dhist;
+  int32_t* dcnt;
+  int32_t* dfl;
+  int32_t* walk_list;
+  int32_t* walk_cnt;
+  int32_t* walk_T;
+  int32_t dK, dpad;
 } gsbn_pop_proj;
 
 /* Delay queue in push form (Proj.cpp:645-664 with ProcExchangeSpike's
@@ -195,8 +210,24 @@ typedef struct {
   gsbn_syn16_rec* syn16;
   float* wij;
   float p_scale, e_scale; /* GSBN_PREC_F16 only; powers of two */
+  /* deferred column updates (dK > 0): the pending column kicks of steps up
+   * to `target` are applied first, as the column updates of those steps
+   * (slots with ii >= 0) */
+  const int32_t* ii;
+  const int32_t* dhist;
+  const int32_t* dcnt;
+  int32_t dK;
+  float kftj;
 } gsbn_all_args;
 int gsbn_proj_all(const gsbn_all_args* a, void* stream);
+
+/* Deferred column updates: apply every pending column kick (steps up to
+ * `target`) to every synapse of a slot with ii >= 0, leaving the state the
+ * reference's eager column updates would have left (before host reads of
+ * the synapse state, snapshots and host writes).  Same arguments as
+ * gsbn_proj_all; table = the current learning rate's; kp, wgain, eps, eps2,
+ * pj and rowst are not used. */
+int gsbn_proj_materialize(const gsbn_all_args* a, void* stream);
 
 /* Postsynaptic Euler update and bias (Proj.cpp:310-363), standalone form. */
 typedef struct {
@@ -244,6 +275,16 @@ typedef struct {
   gsbn_syn16_rec* syn16;
   float* wij;
   float p_scale, e_scale; /* GSBN_PREC_F16 only; powers of two */
+  /* deferred column updates (dK > 0, see gsbn_pop_proj): a row update of
+   * synapse (r, j) first replays the pending column kicks of unit (h, j)
+   * (ring steps s with tij < s < simstep, oldest first, each the column
+   * update of step s); the column pass walks the units of spk_list /
+   * spk_cnt (then the walk lists), replaying their kicks up to simstep from
+   * the ring state walk_T[entry] */
+  const int32_t* dhist;
+  const int32_t* dcnt;
+  const int32_t* walk_T;
+  int32_t dK, dpad;
 } gsbn_rowcol_args;
 int gsbn_proj_rowcol(const gsbn_rowcol_args* a, void* stream);
 

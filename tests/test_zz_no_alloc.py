@@ -27,7 +27,9 @@ class PopProj(ctypes.Structure):
     _fields_ = [("contrib", vp), ("pend_cnt", vp), ("C", i32), ("pj", vp), ("ej", vp),
                 ("zj", vp), ("taupdt", f32), ("ke", f32), ("kzj", f32), ("kepsc", f32),
                 ("kftj", f32), ("bgain", f32), ("eps", f32), ("fut", vp), ("W", i32),
-                ("rows_list", vp), ("rows_cnt", vp), ("act_stamp", vp)]
+                ("rows_list", vp), ("rows_cnt", vp), ("act_stamp", vp), ("dhist", vp),
+                ("dcnt", vp), ("dfl", vp), ("walk_list", vp), ("walk_cnt", vp), ("walk_T", vp),
+                ("dK", i32), ("dpad", i32)]
 
 
 class Fanout(ctypes.Structure):
