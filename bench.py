@@ -72,7 +72,7 @@ def workload(name: str):
     raise SystemExit("unknown workload " + name)
 
 
-DEFERRED = 0  # --deferred K, applied with the plan to every measured spec
+DEFERRED = None  # --deferred K, applied with the plan to every measured spec
 
 
 def apply_plan(spec, plan: str):
@@ -519,9 +519,9 @@ def main():
     ap.add_argument("--plan", default="auto",
                     help="step plan: auto (tuned online), split:N (N row/column workgroups "
                          "per HCU), fused, chained, or wave (batches: one wave per HCU)")
-    ap.add_argument("--deferred", type=int, default=0, metavar="K",
+    ap.add_argument("--deferred", type=int, default=None, metavar="K",
                     help="deferred column updates with rings of K spike steps per unit "
-                         "(ProcUpdLazy deferred-columns; 0: eager)")
+                         "(ProcUpdLazy deferred-columns; 0: eager; default: the library's, 8)")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--cpu-arith", default="ref32", choices=("ref32", "table"))
     ap.add_argument("--step-times", type=int, default=0,

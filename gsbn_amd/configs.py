@@ -146,8 +146,9 @@ class NetSpec:
     # hypercolumn (plan -20); 0: chosen online
     rowcol_wave: int = 0
     # deferred_columns K (a power of two, 2..64): column updates deferred
-    # through rings of K spike steps per postsynaptic unit; 0: eager
-    deferred_columns: int = 0
+    # through rings of K spike steps per postsynaptic unit; 0: eager; None:
+    # ProcUpdLazy's default (8 where it applies)
+    deferred_columns: Optional[int] = None
 
     @property
     def total_hcu(self) -> int:
@@ -315,7 +316,7 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('  argi { key: "rowcol-graded" val: %d }' % spec.rowcol_graded)
     if spec.rowcol_wave:
         L.append('  argi { key: "rowcol-wave" val: %d }' % spec.rowcol_wave)
-    if spec.deferred_columns:
+    if spec.deferred_columns is not None:
         L.append('  argi { key: "deferred-columns" val: %d }' % spec.deferred_columns)
     L.append("}")
     if spec.conn_arrays is None:

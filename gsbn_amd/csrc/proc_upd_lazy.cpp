@@ -1277,8 +1277,10 @@ class ProcUpdLazy : public ProcedureBase {
    * hypercolumn (plan -20) */
   int _wave_fixed = 0;
   /* deferred-columns K: column updates deferred through rings of K spike
-   * steps per unit (0: eager); see Proj::dK */
-  int _defer_k = 0;
+   * steps per unit (0: eager); see Proj::dK.  Default 8 where it applies
+   * (same results, a fraction of the column traffic). */
+  static constexpr int kDeferDefault = 8;
+  int _defer_k = kDeferDefault;
   void end_deferral();
   bool _tuned = false;
   int _tune_pos = 0, _run_len = 0;
@@ -1668,10 +1670,9 @@ void ProcUpdLazy::update_gpu() {
         if (p->_in.size() != 1) ok = false;
       if (!ok) _defer_k = 0;
     }
+    /* the fused launch does eager column updates: asking for it asks for them */
+    if (_fused_fixed) _defer_k = 0;
     if (_defer_k) {
-      if (_fused_fixed)
-        throw Error(-2, "ProcUpdLazy: fused-step runs eager column updates only "
-                        "(deferred-columns 0)");
       _fusable = false;
       for (auto* q : _list_proj) {
         q->enable_deferred(_defer_k);

@@ -33,7 +33,7 @@ def _run(spec, tmp_path, max_steps=None, check_every=0):
     return gpu, ora
 
 
-@pytest.mark.parametrize("K", [2, 4, 16])
+@pytest.mark.parametrize("K", [0, 2, 4, 16])
 def test_config0_full_run(tmp_path, K):
     spec = configs.config0()
     spec.deferred_columns = K
@@ -91,11 +91,11 @@ def test_reference_sparse_delay_map(tmp_path):
     gpu.close()
 
 
-@pytest.mark.parametrize("wave", [0, 1])
-def test_batched_networks(tmp_path, wave):
+@pytest.mark.parametrize("wave,K", [(0, 4), (1, 4), (0, 0)])
+def test_batched_networks(tmp_path, wave, K):
     spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=(3, 140), n_pat=4, train=40,
                                 quiet=5, recall=30, reset=20)
-    spec.deferred_columns = 4
+    spec.deferred_columns = K
     spec.rowcol_wave = wave
     gpu, ora = _run(spec, tmp_path)
     c, t, flags = gpu.check()
@@ -103,7 +103,7 @@ def test_batched_networks(tmp_path, wave):
     gpu.close()
 
 
-@pytest.mark.parametrize("form,K", [("fp32", 8), ("bf16", 8), ("fp16", 4)])
+@pytest.mark.parametrize("form,K", [("fp32", 8), ("bf16", 8), ("fp16", 4), ("fp32", 0)])
 def test_config1_through_first_sweep(tmp_path, form, K):
     # 100 x 100, 300 steps while the tuner alternates split and chained
     # plans, state every 100 steps
@@ -190,11 +190,24 @@ def test_structural_plasticity_ends_deferral(tmp_path):
     gpu.close()
 
 
-def test_fused_step_refused(tmp_path):
+def test_fused_step_runs_eager(tmp_path):
+    # the fused launch does eager column updates: forcing it turns deferral off
     spec = configs.config0()
     spec.deferred_columns = 4
     spec.fused_step = 1
-    s = Solver(configs.write(spec, str(tmp_path)))
-    with pytest.raises(Exception, match="deferred-columns"):
-        s.step(1)
-    s.close()
+    gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path, max_steps=300)
+    assert div is None
+    assert gpu.glob("deferred-columns") == 0 and gpu.glob("rowcol-parts") == -1
+    gpu.close()
+
+
+def test_default_is_deferred(tmp_path):
+    # without the argument ProcUpdLazy defers with rings of 8; 0 asks for eager
+    spec = configs.config0()
+    gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path, max_steps=30)
+    assert div is None and gpu.glob("deferred-columns") == 8
+    gpu.close()
+    spec.deferred_columns = 0
+    gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path, max_steps=30)
+    assert div is None and gpu.glob("deferred-columns") == 0
+    gpu.close()

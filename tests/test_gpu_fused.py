@@ -75,9 +75,12 @@ def test_fused_batched_networks(tmp_path):
 
 def test_online_plan_choice(tmp_path):
     # the tuner tries the split plans and the fused one from step 20 on
-    # (results do not depend on the plan) and keeps one
+    # (results do not depend on the plan) and keeps one; eager column
+    # updates, which the fused launch needs
     spec = configs.config0()
+    spec.deferred_columns = 0
     gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path, max_steps=300)
+    assert gpu.glob("step-plan-ms-1", "f") > 0.0  # the fused plan was timed
     assert div is None, "spike rasters diverge at step %s" % (div,)
     assert not helpers.compare_state(gpu, ora, spec)
     assert gpu.glob("rowcol-parts") != 0
