@@ -277,6 +277,7 @@ def measure(spec, steps, warmup, step_times=0):
         _steps(s, warmup)
         s.sync()
         parts = s.glob("rowcol-parts")
+        deferred = s.glob("deferred-columns")
         t0 = time.perf_counter()
         done = s.step(steps)
         s.sync()
@@ -299,7 +300,7 @@ def measure(spec, steps, warmup, step_times=0):
         k0 = r.work()
         _steps(r, lag)
         r.timing(-1, collect=False)
-        r.timing((1 << 1) | (1 << 6), collect=False)
+        r.timing((1 << 0) | (1 << 1) | (1 << 6), collect=False)
         _steps(r, steps - lag)
         k1 = r.work()
         _steps(r, lag)
@@ -313,26 +314,47 @@ def measure(spec, steps, warmup, step_times=0):
     sweep = int(k1[2] - k0[2])
     bpu = BYTES_PER_SYN_UPDATE[spec.trace_precision]
     # one launch per step (the fused plan, -1, or the chained plan, -2) is
-    # timed as kind 6; split plans time their row/column launches (kind 1)
+    # timed as kind 6; split plans time the population launches (kind 0) and
+    # the row/column launches (kind 1)
     one_launch = parts in (-1, -2)
+    batch = len(spec.pops) > 1
+    rc_name = (("chained_kernel" if parts == -2 else "step_kernel_batch" if batch else "step_kernel")
+               if one_launch else "rowcol_kernel_batch_wave" if parts == -20
+               else "rowcol_kernel_graded" if parts <= -10
+               else "rowcol_kernel_batch" if batch else "rowcol_kernel")
     kind = 6 if one_launch else 1
-    rc_launch, rc_ms = int(launches[kind]), float(ms[kind])
     other = int(launches[1]) if one_launch else int(launches[6])
     if other:
         raise SystemExit("bench: %d launches of another plan in the kernel window" % other)
-    avg_ms = rc_ms / max(rc_launch, 1)
-    bytes_per_launch = bpu * (rows + cols) / max(rc_launch, 1)
+    kernels = {}
+    n_rc = int(launches[kind])
+    rc_bytes = bpu * (rows + cols) / max(n_rc, 1)
     if one_launch:
-        bytes_per_launch += POP_BYTES_PER_UNIT * spec.total_units
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        rc_bytes += POP_BYTES_PER_UNIT * spec.total_units
+    kernels[rc_name] = (n_rc, float(ms[kind]), rc_bytes)
+    if not one_launch and int(launches[0]):
+        kernels["pop_kernel_batch" if batch else "pop_kernel"] = (
+            int(launches[0]), float(ms[0]), float(POP_BYTES_PER_UNIT * spec.total_units))
+    total_ms = sum(v[1] for v in kernels.values())
+    table = {}
+    for name, (n, tot, b) in kernels.items():
+        avg = tot / max(n, 1)
+        gbs = b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+        table[name] = {"launches": n, "avg_launch_ms": avg, "bytes_per_launch": b,
+                       "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                       "share_of_kernel_time": tot / total_ms if total_ms > 0 else None}
+    dominant = max(kernels, key=lambda k: kernels[k][1])
+    d = table[dominant]
     return {"elapsed": el, "done": int(done), "rows": rows, "cols": cols, "sweep": sweep,
             "updates_per_s": (rows + cols + sweep) / el, "steps_per_s": done / el,
             "ms_per_step": 1e3 * el / max(done, 1), "init_s": t_init, "step_ms": step_ms,
-            "rc_launches": rc_launch, "rc_avg_ms": avg_ms, "rc_bytes_per_launch": bytes_per_launch,
-            "rc_achieved_gbs": achieved, "rc_parts": parts if parts > 0 else None,
-            "rc_plan": plan_name(parts), "tune_steps": tune_steps,
-            "rc_kernel": ("chained_kernel" if parts == -2 else "step_kernel") if one_launch
-            else "rowcol_kernel_batch_wave" if parts == -20 else "rowcol_kernel",
+            "rc_launches": table[rc_name]["launches"], "rc_avg_ms": table[rc_name]["avg_launch_ms"],
+            "rc_bytes_per_launch": rc_bytes, "rc_achieved_gbs": table[rc_name]["achieved_gbs"],
+            "rc_parts": parts if parts > 0 else None, "rc_plan": plan_name(parts),
+            "tune_steps": tune_steps, "rc_kernel": rc_name, "kernels": table,
+            "dominant": dominant, "dom_avg_ms": d["avg_launch_ms"],
+            "dom_bytes_per_launch": d["bytes_per_launch"], "dom_achieved_gbs": d["achieved_gbs"],
+            "dom_launches": d["launches"], "deferred": deferred,
             "kernel_window": "steps %d..%d, replayed with events on every launch"
                              % (warmup + 1, warmup + steps)}
 
@@ -442,7 +464,8 @@ def batched(steps: int, warmup: int) -> dict:
             "step_ms_p50": m["step_ms"]["p50"], "step_ms_p99": m["step_ms"]["p99"],
             "rowcol_avg_launch_ms": m["rc_avg_ms"], "rowcol_achieved_gbs": m["rc_achieved_gbs"],
             "rowcol_workgroups_per_hcu": m["rc_parts"], "step_plan": m["rc_plan"],
-            "timed_kernel": m["rc_kernel"]}
+            "timed_kernel": m["rc_kernel"], "deferred_columns": m["deferred"],
+            "kernels": m["kernels"]}
 
 
 def frozen_recall(batch: int = 8192, steps: int = 50, clamp_frac: float = 0.4) -> dict:
@@ -546,14 +569,16 @@ def main():
     # to algorithmic bytes that the counter runs of this kernel and plan
     # measured, applied to this run's algorithmic bytes per launch
     traffic, traffic_ratio = None, None
-    tp = os.path.join(ROOT, "profiles", "rowcol_%s_traffic.json" % args.workload)
+    tp = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tp):
         with open(tp) as f:
-            rec = json.load(f).get("kernels", {}).get(m["rc_kernel"])
-        if rec and rec.get("plan") == m["rc_plan"] and rec.get("algorithmic_bytes_per_launch"):
+            rec = json.load(f).get("kernels", {}).get(m["dominant"])
+        if (rec and rec.get("plan") in (m["rc_plan"], "any") and
+                rec.get("deferred_columns") == m["deferred"] and
+                rec.get("algorithmic_bytes_per_launch")):
             traffic_ratio = rec["hbm_bytes_per_launch"] / rec["algorithmic_bytes_per_launch"]
-            traffic = int(round(traffic_ratio * m["rc_bytes_per_launch"]))
-    achieved = m["rc_achieved_gbs"]
+            traffic = int(round(traffic_ratio * m["dom_bytes_per_launch"]))
+    achieved = m["dom_achieved_gbs"]
     line = {
         "metric": METRIC,
         "value": m["updates_per_s"],
@@ -579,11 +604,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_ratio": traffic_ratio,
-                     "kernel": m["rc_kernel"], "avg_launch_ms": m["rc_avg_ms"],
-                     "launches": m["rc_launches"], "bytes_per_launch": m["rc_bytes_per_launch"],
+                     "kernel": m["dominant"], "avg_launch_ms": m["dom_avg_ms"],
+                     "launches": m["dom_launches"], "bytes_per_launch": m["dom_bytes_per_launch"],
                      "window": m["kernel_window"],
                      "workgroups_per_hcu": m["rc_parts"], "step_plan": m["rc_plan"],
-                     "plan_tuning_steps": m["tune_steps"]},
+                     "deferred_columns": m["deferred"],
+                     "plan_tuning_steps": m["tune_steps"], "kernels": m["kernels"]},
     }
     if m["step_ms"] is not None:
         line["step_ms"] = m["step_ms"]
@@ -602,7 +628,8 @@ def main():
                            "rowcol_avg_launch_ms": f["rc_avg_ms"],
                            "rowcol_achieved_gbs": f["rc_achieved_gbs"],
                            "timed_kernel": f["rc_kernel"], "step_plan": f["rc_plan"],
-                           "bytes_per_update": BYTES_PER_SYN_UPDATE[form]}
+                           "bytes_per_update": BYTES_PER_SYN_UPDATE[form],
+                           "kernels": f["kernels"]}
         line["forms"] = forms
     if args.workload == "100x100" and not args.no_batched:
         line["batched"] = batched(300, 100)

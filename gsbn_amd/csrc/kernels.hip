@@ -27,6 +27,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/gsbn_amd.h"
 #include "detmath.h"
 #include "gsbn_core.h"
@@ -43,6 +45,23 @@ static_assert(sizeof(gsbn_syn16_rec) == 16 && sizeof(gsbn_syn16_t) == 16,
 #define RC_COLS 4 /* column slots per thread and pass */
 #define ALL_THREADS 256
 #define ALL_ROWS 16
+
+namespace {
+/* One scalar load from every 64-byte line of the first BYTES bytes of the
+ * kernel's argument segment, waited for together.  The argument structs are
+ * larger than the scalar registers, so the compiler reads them in several
+ * dependent rounds of scalar loads; each round that touches a line not yet in
+ * the scalar cache costs a trip to L2 or memory.  Touching every line first
+ * makes those trips overlap in one round; the later rounds hit. */
+template <int BYTES>
+__device__ __forceinline__ void touch_kernarg_lines() {
+  const uint32_t* k = (const uint32_t*)__builtin_amdgcn_kernarg_segment_ptr();
+  uint32_t acc = 0;
+#pragma unroll
+  for (int o = 0; o < BYTES / 4; o += 16) acc ^= __builtin_nontemporal_load(k + o);
+  asm volatile("" ::"s"(acc));
+}
+}  // namespace
 
 namespace gsbn {
 /* events for the next timed launch: recorded by the kernel's own dispatch
@@ -516,7 +535,14 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
 
   /* 0. loads that do not depend on this step's results */
   float dsup_old = 0.0f, lg = 0.0f, e0 = 0.0f, bj0 = 0.0f, pj0 = 0.0f, ej0 = 0.0f, zj0 = 0.0f;
-  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f; /* first pending contributions */
+  /* pending contributions read before their count is known (speculative
+   * reads cost traffic when few rows are pending, as in batches; the rest
+   * is read eight rows per round trip below) */
+  constexpr int NPRE = 4;
+  float cpre[NPRE];
+#pragma unroll
+  for (int q = 0; q < NPRE; ++q) cpre[q] = 0.0f;
+  int dcnt0 = 0, dfl0 = 0; /* deferred column updates: this unit's ring state */
   int32_t counter_old = 0;
   uint32_t fut0 = 0u, fut1 = 0u, fut2 = 0u, fut3 = 0u; /* words j + i * blockDim of the ring slot */
   int pend0 = 0;
@@ -542,13 +568,17 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
       pj0 = p0.pj[idx];
       ej0 = p0.ej[idx];
       zj0 = p0.zj[idx];
-      /* rows 0..3 of the hypercolumn's contribution block (inside the block
-       * whatever the count; entries at or beyond the count are not used) */
+      /* rows 0..NPRE-1 of the hypercolumn's contribution block (inside the
+       * block whatever the count; entries at or beyond the count are not
+       * used) */
       const float* c = p0.contrib + ((int64_t)h * p0.C) * M + j;
-      if (p0.C > 0) c0 = c[0];
-      if (p0.C > 1) c1 = c[M];
-      if (p0.C > 2) c2 = c[2 * M];
-      if (p0.C > 3) c3 = c[3 * M];
+#pragma unroll
+      for (int q = 0; q < NPRE; ++q)
+        if (p0.C > q) cpre[q] = c[(int64_t)q * M];
+      if (p0.dK) {
+        dcnt0 = p0.dcnt[idx];
+        dfl0 = p0.dfl[idx];
+      }
     }
     const uint32_t* wslot = p0.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * p0.W;
     if (j < p0.W) fut0 = wslot[j];
@@ -621,18 +651,17 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     const float* c = p.contrib + ((int64_t)h * p.C) * M + j;
     int k = 0;
     if (m == 0) {
-      const float pre[4] = {c0, c1, c2, c3};
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (q < cnt && gsbn_fbits(pre[q]) != GSBN_NO_CONTRIB_BITS) e = e + pre[q];
-      k = 4;
+      for (int q = 0; q < NPRE; ++q)
+        if (q < cnt && gsbn_fbits(cpre[q]) != GSBN_NO_CONTRIB_BITS) e = e + cpre[q];
+      k = NPRE;
     }
-    for (; k < cnt; k += 4) {
-      float v[4];
+    for (; k < cnt; k += 8) {
+      float v[8];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = k + q < cnt ? c[(int64_t)(k + q) * M] : 0.0f;
+      for (int q = 0; q < 8; ++q) v[q] = k + q < cnt ? c[(int64_t)(k + q) * M] : 0.0f;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 8; ++q)
         if (k + q < cnt && gsbn_fbits(v[q]) != GSBN_NO_CONTRIB_BITS) e = e + v[q];
     }
     efin[m] = e;
@@ -769,22 +798,30 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   /* 6. postsynaptic traces and bias for every incoming projection
    * (update_jxx_kernel_cpu, Proj.cpp:310-363); epsc keeps the finalised sum
    * when the traces are not updated */
+  /* (the first projection's fields are read from the kernel arguments, not
+   * through a pointer into device memory: a load there would wait for every
+   * store issued so far) */
+  if (live && nin > 0) {
+    const gsbn_pop_proj& p = a.proj0;
+    if (!a.do_jxx) {
+      a.epsc[idx] = efin[0];
+    } else {
+      a.epsc[idx] = efin[0] * (1 - p.kepsc);
+      if (spk > 0) zj_n += p.kftj;
+      if (bj_w) a.bj[idx] = bj_n;
+      p.pj[idx] = pj_n;
+      p.ej[idx] = ej_n;
+      p.zj[idx] = zj_n;
+    }
+  }
   if (live) {
-    for (int m = 0; m < nin; ++m) {
-      const gsbn_pop_proj& p = m == 0 ? a.proj0 : a.projs[m];
+    for (int m = 1; m < nin; ++m) {
+      const gsbn_pop_proj& p = a.projs[m];
       if (!a.do_jxx) {
         a.epsc[m * N + idx] = efin[m];
         continue;
       }
       a.epsc[m * N + idx] = efin[m] * (1 - p.kepsc);
-      if (m == 0) {
-        if (spk > 0) zj_n += p.kftj;
-        if (bj_w) a.bj[idx] = bj_n;
-        p.pj[idx] = pj_n;
-        p.ej[idx] = ej_n;
-        p.zj[idx] = zj_n;
-        continue;
-      }
       const float kp = p.taupdt * a.prn;
       float pj = p.pj[idx], ej = p.ej[idx], zj = p.zj[idx];
       pj += (ej - pj) * kp;
@@ -806,10 +843,10 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     const int K = p0.dK;
     int wflag = 0, wT = 0;
     if (live && spk) {
-      const int n = p0.dcnt[idx];
+      const int n = dcnt0;
       p0.dhist[(int64_t)idx * K + (n & (K - 1))] = a.simstep;
       p0.dcnt[idx] = n + 1;
-      if (n + 1 - p0.dfl[idx] >= K - 1) {
+      if (n + 1 - dfl0 >= K - 1) {
         p0.dfl[idx] = n + 1;
         wflag = 1;
         wT = n + 1;
@@ -830,8 +867,10 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
 
   /* 7. active rows of this step (Proj.cpp:621-630): the ring slot of this
    * exec step, listed in ascending order and cleared for its next use */
-  for (int m = 0; m < a.nproj_in; ++m) {
-    const gsbn_pop_proj& p = m == 0 ? a.proj0 : a.projs[m];
+  /* a generic lambda: the first projection's call (fields in the kernel
+   * arguments) and the others' (fields in device memory) are separate code */
+  auto active_rows = [&](const gsbn_pop_proj& p, const int m, auto first_tag) {
+    (void)first_tag;
     const int C = p.C;
     const int W = p.W;
     uint32_t* words = p.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * W;
@@ -884,10 +923,13 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     /* the next projection reuses the bitmap; after the last one the kernel
      * ends, or the fused caller synchronises before reading the lists */
     if (m + 1 < a.nproj_in) __syncthreads();
-  }
+  };
+  if (nin > 0) active_rows(a.proj0, 0, std::true_type{});
+  for (int m = 1; m < nin; ++m) active_rows(a.projs[m], m, std::false_type{});
 }
 
 __global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
+  touch_kernarg_lines<sizeof(gsbn_pop_args)>();
   pop_body<false>(a, blockIdx.x);
 }
 
@@ -1135,6 +1177,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
 
 template <int PREC>
 __global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC)) rowcol_kernel(gsbn_rowcol_args a) {
+  touch_kernarg_lines<sizeof(gsbn_rowcol_args)>();
   const int h = blockIdx.x / a.parts;
   const RcStep q = {a.simstep, a.exec, a.parts, a.table_len, a.kp, a.table};
   rowcol_body<PREC>(a, q, h, blockIdx.x - h * a.parts);
