@@ -61,6 +61,17 @@ __device__ __forceinline__ void touch_kernarg_lines() {
   for (int o = 0; o < BYTES / 4; o += 16) acc ^= __builtin_nontemporal_load(k + o);
   asm volatile("" ::"s"(acc));
 }
+
+/* The same for an argument struct in device memory (a batched launch's
+ * per-network arguments; the address must be uniform across the wave). */
+template <int BYTES>
+__device__ __forceinline__ void touch_lines(const void* p) {
+  const uint32_t* k = static_cast<const uint32_t*>(p);
+  uint32_t acc = 0;
+#pragma unroll
+  for (int o = 0; o < BYTES / 4; o += 16) acc ^= __builtin_nontemporal_load(k + o);
+  asm volatile("" ::"s"(acc));
+}
 }  // namespace
 
 namespace gsbn {
@@ -497,6 +508,27 @@ __device__ __forceinline__ float chained_rows(const gsbn_rowcol_args& b, const R
   return e;
 }
 
+/* The per-step values of a population update: the argument struct's own
+ * fields for a single launch, the batch's shared step fields (with the
+ * stimulus and mask rows offset) for a batched one, so that a batched
+ * workgroup can read its population's arguments in place, from device
+ * memory, instead of copying them into registers. */
+struct PopStep {
+  int simstep, exec, do_jxx;
+  float prn;
+  const float* lginp;
+  const float* wmask;
+};
+
+__device__ __forceinline__ PopStep pop_step(const gsbn_pop_args& a) {
+  return {a.simstep, a.exec, a.do_jxx, a.prn, a.lginp, a.wmask};
+}
+
+__device__ __forceinline__ PopStep pop_step(const gsbn_pop_args& a, const gsbn_step_args& st) {
+  return {st.simstep, st.exec, st.do_jxx, st.prn, a.lginp + st.lginp_offset,
+          a.wmask + st.wmask_offset};
+}
+
 /* One hypercolumn's population update.  FUSED: `f` also receives what the
  * fused step's row/column phase needs (one incoming projection).
  *
@@ -507,7 +539,7 @@ __device__ __forceinline__ float chained_rows(const gsbn_rowcol_args& b, const R
  * store of the update is issued after the last load: loads and stores share
  * one in-order counter, so a load issued behind stores would wait for them. */
 template <bool FUSED, int CPREC = -1>
-__device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
+__device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& ps, const int h,
                                          FusedLds* f = nullptr,
                                          const gsbn_rowcol_args* rb = nullptr,
                                          const RcStep* rq = nullptr, const int has_prev = 0) {
@@ -546,10 +578,10 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   int32_t counter_old = 0;
   uint32_t fut0 = 0u, fut1 = 0u, fut2 = 0u, fut3 = 0u; /* words j + i * blockDim of the ring slot */
   int pend0 = 0;
-  const float wmv = a.wmask[h];
+  const float wmv = ps.wmask[h];
   if (live) {
     dsup_old = a.dsup[idx];
-    lg = a.lginp[idx];
+    lg = ps.lginp[idx];
     counter_old = a.counter[idx];
   }
   /* fan-out range of this unit in the first outgoing projection (the queue
@@ -580,7 +612,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
         dfl0 = p0.dfl[idx];
       }
     }
-    const uint32_t* wslot = p0.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * p0.W;
+    const uint32_t* wslot = p0.fut + ((int64_t)(ps.exec % GSBN_FUT) * a.H + h) * p0.W;
     if (j < p0.W) fut0 = wslot[j];
     if (j + bd < p0.W) fut1 = wslot[j + bd];
     if (j + 2 * bd < p0.W) fut2 = wslot[j + 2 * bd];
@@ -601,27 +633,27 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     if (live) sjp = rb->sj[idx];
   }
   const float d0 = a.wta_pop_max ? a.scratch_dsup0[0] : 0.0f;
-  const bool ext_on = a.ext_off && a.simstep >= 0 && a.simstep < a.ext_nsteps;
+  const bool ext_on = a.ext_off && ps.simstep >= 0 && ps.simstep < a.ext_nsteps;
   int ext_b = 0, ext_e = 0;
   if (ext_on) {
-    ext_b = a.ext_off[a.simstep];
-    ext_e = a.ext_off[a.simstep + 1];
+    ext_b = a.ext_off[ps.simstep];
+    ext_e = a.ext_off[ps.simstep + 1];
   }
   /* the draws and the first projection's postsynaptic traces do not depend
    * on this step's results either: computed while the loads are in flight */
   float u = 0.0f, xi = 0.0f;
   if (live) {
     float z;
-    gsbn_draw_unit(a.seed, (uint32_t)a.pop_id, (uint32_t)a.simstep, (uint32_t)idx, &u, &z);
+    gsbn_draw_unit(a.seed, (uint32_t)a.pop_id, (uint32_t)ps.simstep, (uint32_t)idx, &u, &z);
     xi = a.snoise * z;
   }
   float pj_n = 0.0f, ej_n = 0.0f, zj_n = 0.0f, bj_n = 0.0f;
   bool bj_w = false;
-  if (a.do_jxx && live && nin > 0) {
+  if (ps.do_jxx && live && nin > 0) {
     /* update_jxx_kernel_cpu (Proj.cpp:310-363) for projection 0; the spike
      * increment of zj follows in step 6 */
     const gsbn_pop_proj& p0 = a.proj0;
-    const float kp = p0.taupdt * a.prn;
+    const float kp = p0.taupdt * ps.prn;
     pj_n = pj0 + (ej0 - pj0) * kp;
     ej_n = ej0 + (zj0 - ej0) * p0.ke;
     zj_n = zj0 * (1 - p0.kzj);
@@ -759,7 +791,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
       sh_f[pos_s] = __int_as_float(fo1);
     }
   }
-  if (FUSED && live) sh_pj[j] = a.do_jxx ? pj_n : pj0;
+  if (FUSED && live) sh_pj[j] = ps.do_jxx ? pj_n : pj0;
 
   /* 5b. queue push of this step's spikes (what the queue update of the next
    * step would push from si, Proj.cpp:645-664) into the outgoing rings; its
@@ -767,10 +799,10 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   if (a.nproj_out > 0 && spk_tot > 0) {
     __syncthreads();
     for (int i = 0; i < spk_tot; ++i)
-      push_rows(a.out0, sh_ext[i], __float_as_int(sh_f[i]), a.exec, threadIdx.x, blockDim.x);
+      push_rows(a.out0, sh_ext[i], __float_as_int(sh_f[i]), ps.exec, threadIdx.x, blockDim.x);
     for (int o = 1; o < a.nproj_out; ++o)
       for (int i = 0; i < spk_tot; ++i)
-        push_unit(a.outs[o], h * M + sh_spk[i], a.exec, threadIdx.x, blockDim.x);
+        push_unit(a.outs[o], h * M + sh_spk[i], ps.exec, threadIdx.x, blockDim.x);
   }
 
   /* 7a. the first projection's ring slot of this exec step into LDS (from
@@ -778,7 +810,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
    * reads the rest here, before the stores) */
   if (nin > 0) {
     const int W = a.proj0.W;
-    const uint32_t* words = a.proj0.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * W;
+    const uint32_t* words = a.proj0.fut + ((int64_t)(ps.exec % GSBN_FUT) * a.H + h) * W;
     for (int w = j, i = 0; w < W; w += bd, ++i)
       sh_bitmap[w] = i == 0 ? fut0 : i == 1 ? fut1 : i == 2 ? fut2 : i == 3 ? fut3 : words[w];
   }
@@ -803,7 +835,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
    * store issued so far) */
   if (live && nin > 0) {
     const gsbn_pop_proj& p = a.proj0;
-    if (!a.do_jxx) {
+    if (!ps.do_jxx) {
       a.epsc[idx] = efin[0];
     } else {
       a.epsc[idx] = efin[0] * (1 - p.kepsc);
@@ -817,12 +849,12 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
   if (live) {
     for (int m = 1; m < nin; ++m) {
       const gsbn_pop_proj& p = a.projs[m];
-      if (!a.do_jxx) {
+      if (!ps.do_jxx) {
         a.epsc[m * N + idx] = efin[m];
         continue;
       }
       a.epsc[m * N + idx] = efin[m] * (1 - p.kepsc);
-      const float kp = p.taupdt * a.prn;
+      const float kp = p.taupdt * ps.prn;
       float pj = p.pj[idx], ej = p.ej[idx], zj = p.zj[idx];
       pj += (ej - pj) * kp;
       ej += (zj - ej) * p.ke;
@@ -844,7 +876,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     int wflag = 0, wT = 0;
     if (live && spk) {
       const int n = dcnt0;
-      p0.dhist[(int64_t)idx * K + (n & (K - 1))] = a.simstep;
+      p0.dhist[(int64_t)idx * K + (n & (K - 1))] = ps.simstep;
       p0.dcnt[idx] = n + 1;
       if (n + 1 - dfl0 >= K - 1) {
         p0.dfl[idx] = n + 1;
@@ -873,7 +905,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
     (void)first_tag;
     const int C = p.C;
     const int W = p.W;
-    uint32_t* words = p.fut + ((int64_t)(a.exec % GSBN_FUT) * a.H + h) * W;
+    uint32_t* words = p.fut + ((int64_t)(ps.exec % GSBN_FUT) * a.H + h) * W;
     if (m > 0) {
       for (int w = j; w < W; w += bd) sh_bitmap[w] = words[w];
     }
@@ -896,7 +928,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
         bits &= bits - 1;
         const int k = (w << 5) + b;
         p.rows_list[(int64_t)h * C + pos] = k;
-        p.act_stamp[(int64_t)h * C + k] = a.exec;
+        p.act_stamp[(int64_t)h * C + k] = ps.exec;
         if (FUSED) rows[pos] = k;
         ++pos;
       }
@@ -930,7 +962,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const int h,
 
 __global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
   touch_kernarg_lines<sizeof(gsbn_pop_args)>();
-  pop_body<false>(a, blockIdx.x);
+  pop_body<false>(a, pop_step(a), blockIdx.x);
 }
 
 /* per-step fields shared by every population / projection of a batch */
@@ -950,9 +982,9 @@ __global__ void __launch_bounds__(POP_MAX_THREADS)
                      const int32_t* __restrict__ wg_hcu, int nwg, gsbn_step_args st) {
   for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
     if (w != (int)blockIdx.x) __syncthreads(); /* LDS of the previous hypercolumn */
-    gsbn_pop_args a = arr[wg_pop[w]];
-    apply_step(a, st);
-    pop_body<false>(a, wg_hcu[w]);
+    const gsbn_pop_args& a = arr[wg_pop[w]];
+    touch_lines<sizeof(gsbn_pop_args)>(&a);
+    pop_body<false>(a, pop_step(a, st), wg_hcu[w]);
   }
 }
 
@@ -1241,7 +1273,7 @@ template <int PREC>
 __global__ void __launch_bounds__(RC_THREADS) step_kernel(gsbn_pop_args a, gsbn_rowcol_args b) {
   const int h = blockIdx.x;
   FusedLds f;
-  pop_body<true>(a, h, &f);
+  pop_body<true>(a, pop_step(a), h, &f);
   __syncthreads();
   const RcStep q = {b.simstep, b.exec, 1, b.table_len, b.kp, b.table};
   rowcol_body<PREC, true>(b, q, h, 0, &f);
@@ -1254,10 +1286,10 @@ __global__ void __launch_bounds__(RC_THREADS)
                       const int32_t* __restrict__ in_proj, gsbn_step_args st) {
   const int p = wg_pop[blockIdx.x];
   const int h = wg_hcu[blockIdx.x];
-  gsbn_pop_args a = pops[p];
-  apply_step(a, st);
+  const gsbn_pop_args& a = pops[p];
+  touch_lines<sizeof(gsbn_pop_args)>(&a);
   FusedLds f;
-  pop_body<true>(a, h, &f);
+  pop_body<true>(a, pop_step(a, st), h, &f);
   __syncthreads();
   const gsbn_rowcol_args& b = projs[in_proj[p]];
   const RcStep q = {st.simstep, st.exec, 1, b.table_lens[st.kp_slot], b.taupdt * st.prn,
@@ -1280,7 +1312,7 @@ __global__ void __launch_bounds__(RC_THREADS) chained_kernel(gsbn_pop_args a, gs
     /* the population half is the longer path: its waves take issue
      * priority over the column workgroups that share their CUs */
     __builtin_amdgcn_s_setprio(2);
-    pop_body<false, PREC>(a, blockIdx.x, nullptr, &b, &q, has_prev);
+    pop_body<false, PREC>(a, pop_step(a), blockIdx.x, nullptr, &b, &q, has_prev);
     return;
   }
   const int w = blockIdx.x - a.H;
