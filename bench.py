@@ -9,26 +9,36 @@ Synthetic input: seeded one-MCU-per-HCU patterns.  The timed steps are taken
 from the training phase after the warm-up steps (the first step carries the
 full catch-up sweep of the learning-rate switch and is part of the warm-up).
 
+Step plan: the online plan tuner first runs to completion on a solver of its
+own (same network and schedule); the measured solvers then run its choice
+from their first step, so no tuning sample falls inside a timed window.
+
 Printed value: synapse-trace updates per second, counted as the reference's
 loops do (each active row updates M synapses, each active column C synapses,
 plus any full-sweep synapses), with simulated steps per second beside it.
 
-roofline: the row/column kernel, timed with HIP events recorded by each of
-its launches on the solver stream over up to 1000 steps that follow the
-timed region (the timed steps themselves run uninstrumented); algorithmic
-bytes are 44 per synapse update (read pij, eij, zi2, zj2, tij; write them
-and wij).  When the online tuner chose a one-launch-per-step plan (the fused
-step, or the chained step that runs the previous step's row/column updates
-with this step's population update), that kernel is the one timed and its
-algorithmic bytes add 40 per unit for the population update.
+roofline: the kernel with the largest share of kernel time in the timed
+window (the population kernel at 100 x 100), from a second solver that
+replays the same steps with HIP events recorded by every launch (results are
+deterministic, so the replay runs the same work; the timed solver itself
+runs uninstrumented).  Algorithmic bytes: 44 per synapse update for the
+row/column kernel (read pij, eij, zi2, zj2, tij; write them and wij; 36 in
+the 16-bit forms), 73 per unit for the population kernel (which fuses the
+postsynaptic trace update) plus 4 per support contribution it reads.
+`traffic`: HBM bytes per launch from rocprofv3 request counters of the same
+kernel and plan (profiles/traffic_<workload>.json, see gsbn_amd.traffic),
+scaled to this run's launches.  `kernels` lists every timed kernel.
 
-forms: the same workload with Pij/Eij stored in bf16 and in scaled fp16
-(36 algorithmic bytes per update), same steps, reported beside the plain
-FP32 value.
+forms: the same workload with Pij/Eij stored in bf16 and in scaled fp16.
+batched: 512 independent networks in lock-step.  frozen_recall: 8192 cues of
+the trained network, exact and bf16-MFMA support.  hbm_copy: the library's
+copy kernel on 2 GiB, the practical bandwidth ceiling of the box.
 
 cpu_baseline: this project's CPU restatement of the same procedure (the
-oracle, plain C, -O2, one thread pinned to one core) on the same workload:
-the same warm-up, then as many steps as fit in about 20 s.
+oracle, plain C, -O2, one thread pinned to one core) with the reference's
+float closed forms and libm exp/log per update (Proj.cpp:241-277), on the
+same timed steps (`same_window`) and on a bounded sample of the training
+phase after them; the oracle's table arithmetic is timed beside it.
 """
 
 from __future__ import annotations
@@ -52,7 +62,12 @@ METRIC = ("synapse-trace updates/sec + sim steps/sec at 100×100 HCU×MCU, plain
 # and writes them and wij (row) or reads/writes the five and reads ii
 # (column); 16-bit forms move 2-byte pij/eij
 BYTES_PER_SYN_UPDATE = {"fp32": 44, "bf16": 36, "fp16": 36}
-POP_BYTES_PER_UNIT = 40  # population update: dsup, act, epsc, bj, pj, ej, zj, counter, spike
+# population kernel (population update with the postsynaptic traces fused):
+# reads dsup, lginp, counter, epsc, bj, pj, ej, zj (32 B), writes dsup, act,
+# the two draws, counter, epsc, bj, pj, ej, zj (40 B) and the spike (1 B);
+# plus 4 B per support contribution of the previous step's row updates
+POP_BYTES_PER_UNIT = 73
+POP_BYTES_PER_CONTRIB = 4
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense bf16
 
@@ -329,12 +344,16 @@ def measure(spec, steps, warmup, step_times=0):
     kernels = {}
     n_rc = int(launches[kind])
     rc_bytes = bpu * (rows + cols) / max(n_rc, 1)
+    # the one-launch plans sum the support contributions on chip
+    pop_bytes = POP_BYTES_PER_UNIT * spec.total_units
     if one_launch:
-        rc_bytes += POP_BYTES_PER_UNIT * spec.total_units
+        rc_bytes += pop_bytes
+    elif int(launches[0]):
+        pop_bytes += POP_BYTES_PER_CONTRIB * rows / int(launches[0])
     kernels[rc_name] = (n_rc, float(ms[kind]), rc_bytes)
     if not one_launch and int(launches[0]):
         kernels["pop_kernel_batch" if batch else "pop_kernel"] = (
-            int(launches[0]), float(ms[0]), float(POP_BYTES_PER_UNIT * spec.total_units))
+            int(launches[0]), float(ms[0]), float(pop_bytes))
     total_ms = sum(v[1] for v in kernels.values())
     table = {}
     for name, (n, tot, b) in kernels.items():
@@ -357,6 +376,25 @@ def measure(spec, steps, warmup, step_times=0):
             "dom_launches": d["launches"], "deferred": deferred,
             "kernel_window": "steps %d..%d, replayed with events on every launch"
                              % (warmup + 1, warmup + steps)}
+
+
+def traffic_of(workload_name: str, m: dict, kernel: str):
+    """HBM bytes per launch of `kernel` in the run `m`: the ratio of counter
+    bytes to algorithmic bytes that the counter runs of this workload, kernel
+    and plan measured (profiles/traffic_<workload>.json, written by
+    gsbn_amd.traffic), applied to this run's algorithmic bytes per launch.
+    (None, None) when no such measurement exists."""
+    tp = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload_name)
+    if not os.path.exists(tp):
+        return None, None
+    with open(tp) as f:
+        rec = json.load(f).get("kernels", {}).get(kernel)
+    plan = m["rc_plan"] if not kernel.startswith("pop_kernel") else "any"
+    if not (rec and rec.get("plan") == plan and rec.get("deferred_columns") == m["deferred"] and
+            rec.get("algorithmic_bytes_per_launch")):
+        return None, None
+    ratio = rec["hbm_bytes_per_launch"] / rec["algorithmic_bytes_per_launch"]
+    return int(round(ratio * m["kernels"][kernel]["bytes_per_launch"])), ratio
 
 
 def copy_calibration(mib: int):
@@ -457,6 +495,7 @@ def batched(steps: int, warmup: int) -> dict:
     spec, desc = workload("batch512")
     spec.deferred_columns = DEFERRED
     m = measure(spec, steps, warmup, step_times=100)
+    traffic, ratio = traffic_of("batch512", m, m["dominant"])
     n = len(spec.pops)
     return {"workload": desc, "networks": n, "steps": m["done"],
             "steps_per_sec": m["steps_per_s"], "network_steps_per_sec": n * m["steps_per_s"],
@@ -465,7 +504,9 @@ def batched(steps: int, warmup: int) -> dict:
             "rowcol_avg_launch_ms": m["rc_avg_ms"], "rowcol_achieved_gbs": m["rc_achieved_gbs"],
             "rowcol_workgroups_per_hcu": m["rc_parts"], "step_plan": m["rc_plan"],
             "timed_kernel": m["rc_kernel"], "deferred_columns": m["deferred"],
-            "kernels": m["kernels"]}
+            "kernels": m["kernels"], "dominant_kernel": m["dominant"],
+            "dominant_frac": m["dom_achieved_gbs"] / HBM_PEAK_GBS,
+            "dominant_traffic": traffic, "dominant_traffic_ratio": ratio}
 
 
 def frozen_recall(batch: int = 8192, steps: int = 50, clamp_frac: float = 0.4) -> dict:
@@ -564,20 +605,7 @@ def main():
     spec, desc = workload(args.workload)
     apply_plan(spec, args.plan)
     m = measure(spec, args.steps, args.warmup, args.step_times)
-    # HBM bytes per launch of the timed kernel: the ratio of counter bytes
-    # (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy of known size)
-    # to algorithmic bytes that the counter runs of this kernel and plan
-    # measured, applied to this run's algorithmic bytes per launch
-    traffic, traffic_ratio = None, None
-    tp = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
-    if os.path.exists(tp):
-        with open(tp) as f:
-            rec = json.load(f).get("kernels", {}).get(m["dominant"])
-        if (rec and rec.get("plan") in (m["rc_plan"], "any") and
-                rec.get("deferred_columns") == m["deferred"] and
-                rec.get("algorithmic_bytes_per_launch")):
-            traffic_ratio = rec["hbm_bytes_per_launch"] / rec["algorithmic_bytes_per_launch"]
-            traffic = int(round(traffic_ratio * m["dom_bytes_per_launch"]))
+    traffic, traffic_ratio = traffic_of(args.workload, m, m["dominant"])
     achieved = m["dom_achieved_gbs"]
     line = {
         "metric": METRIC,
