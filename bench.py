@@ -30,7 +30,8 @@ kernel and plan (profiles/traffic_<workload>.json, see gsbn_amd.traffic),
 scaled to this run's launches.  `kernels` lists every timed kernel.
 
 forms: the same workload with Pij/Eij stored in bf16 and in scaled fp16.
-batched: 512 independent networks in lock-step.  frozen_recall: 8192 cues of
+batched: 512 independent networks in lock-step (batched_bf16: the same with
+Pij/Eij in bf16).  frozen_recall: 8192 cues of
 the trained network, exact and bf16-MFMA support.  hbm_copy: the library's
 copy kernel on 2 GiB, the practical bandwidth ceiling of the box.
 
@@ -490,10 +491,13 @@ def hbm_copy(gib: int = 2, reps: int = 10) -> dict:
             "copy_checked": ok}
 
 
-def batched(steps: int, warmup: int) -> dict:
+def batched(steps: int, warmup: int, form: str = "fp32") -> dict:
     """The 512-network form: aggregate and per-network step rates."""
     spec, desc = workload("batch512")
     spec.deferred_columns = DEFERRED
+    spec.trace_precision = form
+    if form != "fp32":
+        desc = desc.replace("plain FP32 state", "Pij/Eij stored in " + form)
     m = measure(spec, steps, warmup, step_times=100)
     traffic, ratio = traffic_of("batch512", m, m["dominant"])
     n = len(spec.pops)
@@ -663,6 +667,11 @@ def main():
         line["batched"] = batched(300, 100)
         line["batched"]["speedup_network_steps_per_sec"] = (
             line["batched"]["network_steps_per_sec"] / m["steps_per_s"])
+        if not args.no_forms:
+            # the BF16-state batched form against the plain single network
+            line["batched_bf16"] = batched(300, 100, "bf16")
+            line["batched_bf16"]["speedup_network_steps_per_sec"] = (
+                line["batched_bf16"]["network_steps_per_sec"] / m["steps_per_s"])
     if args.workload == "100x100" and not args.no_recall:
         line["frozen_recall"] = frozen_recall()
     if args.workload == "100x100" and not args.no_forms:

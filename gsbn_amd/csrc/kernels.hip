@@ -25,6 +25,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <limits.h>
 #include <stdint.h>
 
 #include <type_traits>
@@ -371,6 +372,78 @@ __device__ __forceinline__ bool replay_kicks(const SynIO<PREC>& io, SynVals& v,
     io.round_traces(v);
   }
   return n0 <= n1;
+}
+
+/* The same for a ring of K = 8, held in registers: the ring is read with two
+ * 16-byte loads that need no earlier result (issued beside the synapse
+ * record), the entries are put in spike order by register selects, and the
+ * coefficients of each kick are loaded while the previous kick is applied,
+ * instead of one dependent load per ring entry and per kick.
+ * s[q] is the step of spike number T - 7 + q (q = 0..6: the last K - 1
+ * spikes); numbers below 0 hold INT_MIN and are never applied. */
+struct Ring8 {
+  int s[7];
+};
+
+__device__ __forceinline__ Ring8 ring8(const int4 lo, const int4 hi, const int T) {
+  const int e[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  Ring8 r;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int n = T - 7 + q;
+    const int idx = n & 7;
+    int v = e[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) v = idx == i ? e[i] : v;
+    r.s[q] = n >= 0 ? v : INT_MIN;
+  }
+  return r;
+}
+
+__device__ __forceinline__ Ring8 ring8_load(const int32_t* ring, const int T) {
+  const int4* p = reinterpret_cast<const int4*>(ring);
+  return ring8(p[0], p[1], T);
+}
+
+/* replay_kicks for K = 8 from registers: applies the entries with
+ * v.tij < s <= upto in spike order, each as the reference's column update
+ * of step s.  With own != nullptr, also loads (first, beside the first
+ * kick's) the coefficients of the gap from the last applied kick (or tij)
+ * to own_t, when that gap is positive. */
+template <int PREC>
+__device__ __forceinline__ bool replay_ring8(const SynIO<PREC>& io, SynVals& v, const Ring8& r,
+                                             int upto, const float* table, int table_len,
+                                             float kftj, int own_t = 0,
+                                             gsbn_syn_coef_t* own = nullptr) {
+  bool sel[7];
+  int tend = v.tij;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    sel[q] = r.s[q] > v.tij && r.s[q] <= upto;
+    if (sel[q]) tend = r.s[q];
+  }
+  if (own && own_t - tend > 0) *own = load_coef(table, table_len, own_t - tend);
+  /* the selected entries are consecutive (the ring's steps ascend) */
+  gsbn_syn_coef_t cur;
+  if (sel[0]) cur = load_coef(table, table_len, r.s[0] - v.tij);
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const bool next = q + 1 < 7 && sel[q + 1 < 7 ? q + 1 : 6];
+    if (sel[q]) {
+      gsbn_syn_coef_t nxt;
+      if (next) nxt = load_coef(table, table_len, r.s[q + 1 < 7 ? q + 1 : 6] - r.s[q]);
+      gsbn_syn_catchup_k(cur, &v.pij, &v.eij, &v.zi2, &v.zj2);
+      v.zj2 = v.zj2 + kftj;
+      v.tij = r.s[q];
+      io.round_traces(v);
+      any = true;
+      if (next) cur = nxt;
+    } else if (next) {
+      cur = load_coef(table, table_len, r.s[q + 1 < 7 ? q + 1 : 6] - v.tij);
+    }
+  }
+  return any;
 }
 
 /* per-launch values of a row/column update (taken from the argument struct
@@ -1018,7 +1091,7 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_arg
 /* Part `part` of the row/column update of hypercolumn h.  FUSED: the
  * hypercolumn's active rows, spikes and pj come from the population phase
  * of the same workgroup (fl), not from memory. */
-template <int PREC, bool FUSED = false, bool ROWS = true, int NT = RC_THREADS>
+template <int PREC, bool FUSED = false, bool ROWS = true, int NT = RC_THREADS, bool RING8 = false>
 __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcStep& q,
                                             const int h, const int part,
                                             const FusedLds* fl = nullptr) {
@@ -1064,11 +1137,22 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
       const int64_t sidx = r * M + j;
       gsbn_row_rec st = a.rowst[r];
       SynVals s = io.load(sidx, !learn);
-      if (a.dK) {
-        /* deferred column updates: this synapse's pending kicks first */
+      /* the synapse's own coefficients (gap to this step from tij after
+       * any replayed kicks) */
+      gsbn_syn_coef_t cown;
+      if (RING8 && a.dK == 8) {
+        /* deferred column updates: this synapse's pending kicks first, from
+         * the unit's ring in registers; the ring is read beside the record */
         const int uj = h * M + j;
-        replay_kicks<PREC>(io, s, a.dhist + (int64_t)uj * a.dK, a.dcnt[uj], a.dK, t - 1, q.table,
-                           q.table_len, a.kftj);
+        const Ring8 rg = ring8_load(a.dhist + (int64_t)uj * 8, a.dcnt[uj]);
+        replay_ring8<PREC>(io, s, rg, t - 1, q.table, q.table_len, a.kftj, t, &cown);
+      } else {
+        if (a.dK) {
+          const int uj = h * M + j;
+          replay_kicks<PREC>(io, s, a.dhist + (int64_t)uj * a.dK, a.dcnt[uj], a.dK, t - 1,
+                             q.table, q.table_len, a.kftj);
+        }
+        if (t - s.tij > 0) cown = load_coef(q.table, q.table_len, t - s.tij);
       }
       const int Dr = t - st.ti;
       if (Dr <= 0) {
@@ -1091,7 +1175,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
         s.tij = t;
         cval = gsbn_bitsf(GSBN_NO_CONTRIB_BITS);
       } else {
-        gsbn_syn_catchup_k(load_coef(q.table, q.table_len, D), &s.pij, &s.eij, &s.zi2, &s.zj2);
+        gsbn_syn_catchup_k(cown, &s.pij, &s.eij, &s.zi2, &s.zj2);
         s.zi2 = s.zi2 + a.kfti;
         s.tij = t;
         if (learn) {
@@ -1202,17 +1286,21 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   }
 }
 
-/* six waves per SIMD fit the FP32 and BF16 forms without spills (one wave
- * more than their natural register count allows); the FP16 form would
- * spill and keeps five */
+/* six waves per SIMD fit the batched FP32 and BF16 forms without spills
+ * (one wave more than their natural register count allows); the FP16 form
+ * would spill and keeps five */
 #define RC_WAVES(PREC) ((PREC) == GSBN_PREC_F16 ? 5 : 6)
 
+/* The single-network launch replays the deferred kicks of its row updates
+ * from registers (Ring8), which needs more registers: five waves per SIMD,
+ * more than its grid (a few hundred workgroups) fills.  The batched launches
+ * fill the device and keep the register-lean replay and six waves. */
 template <int PREC>
-__global__ void __launch_bounds__(RC_THREADS, RC_WAVES(PREC)) rowcol_kernel(gsbn_rowcol_args a) {
+__global__ void __launch_bounds__(RC_THREADS, 5) rowcol_kernel(gsbn_rowcol_args a) {
   touch_kernarg_lines<sizeof(gsbn_rowcol_args)>();
   const int h = blockIdx.x / a.parts;
   const RcStep q = {a.simstep, a.exec, a.parts, a.table_len, a.kp, a.table};
-  rowcol_body<PREC>(a, q, h, blockIdx.x - h * a.parts);
+  rowcol_body<PREC, false, true, RC_THREADS, true>(a, q, h, blockIdx.x - h * a.parts);
 }
 
 /* work item i = (map entry i / parts, part i % parts); workgroup b takes
