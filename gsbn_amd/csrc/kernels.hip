@@ -170,6 +170,51 @@ __device__ __forceinline__ int block_scan_fast(int v, int* sh, int* total) {
   return base + x - v;
 }
 
+/* Exclusive workgroup prefix sums of three 32-bit values at once (each may
+ * pack two 16-bit counts whose workgroup totals stay below 2^16, so the
+ * fields add without carries); the totals go to tot.  sh: 3 words per wave,
+ * used by this one scan of the launch (no barrier after it). */
+__device__ __forceinline__ void block_scan3(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t* sh,
+                                            uint32_t* tot) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t x = a, y = b, z = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t x1 = __shfl_up(x, o, 64);
+    const uint32_t y1 = __shfl_up(y, o, 64);
+    const uint32_t z1 = __shfl_up(z, o, 64);
+    if (lane >= o) {
+      x += x1;
+      y += y1;
+      z += z1;
+    }
+  }
+  if (lane == 63) {
+    sh[3 * wid] = x;
+    sh[3 * wid + 1] = y;
+    sh[3 * wid + 2] = z;
+  }
+  __syncthreads();
+  uint32_t bx = 0, by = 0, bz = 0, tx = 0, ty = 0, tz = 0;
+  for (int w = 0; w < nw; ++w) {
+    const uint32_t u = sh[3 * w], v = sh[3 * w + 1], q = sh[3 * w + 2];
+    if (w < wid) {
+      bx += u;
+      by += v;
+      bz += q;
+    }
+    tx += u;
+    ty += v;
+    tz += q;
+  }
+  a = bx + x - a;
+  b = by + y - b;
+  c = bz + z - c;
+  tot[0] = tx;
+  tot[1] = ty;
+  tot[2] = tz;
+}
+
 /* epsc of unit (h,j) for incoming projection m after adding the pending row
  * contributions of the previous step, in ascending row order */
 __device__ float finalized_epsc(const gsbn_pop_args& a, const gsbn_pop_proj& p, int m, int h,
@@ -624,6 +669,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
   int* const sh_spk = reinterpret_cast<int*>(sh_dyn + 2 * blockDim.x);
   uint32_t* const sh_bitmap = sh_dyn + 3 * blockDim.x;
   __shared__ int sh_scan[POP_MAX_THREADS / 64 + 1];
+  __shared__ uint32_t sh_scan3[3 * (POP_MAX_THREADS / 64)];
   __shared__ int sh_pend[GSBN_MAX_PROJ_IN];
   __shared__ float sh_wmax[POP_MAX_THREADS / 64]; /* per-wave WTA maxima */
   __shared__ float sh_vsum;
@@ -853,11 +899,39 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
   }
   if (FUSED && live) sh_sj[j] = spk;
 
-  /* 5. spiking units of this hypercolumn, ascending (.ssj, Proj.cpp:632-643) */
+  /* 5. spiking units of this hypercolumn, ascending (.ssj, Proj.cpp:632-643).
+   * Without the fused caller, and when the first projection's ring slot is
+   * in the words read at the start, one scan also places this step's
+   * deferred column walks (step 6b) and its active rows (step 7): packed
+   * counts of spikes | walks, and of set bits in ring words j + i*bd. */
   int spk_tot;
   int pos_s = 0;
-  {
+  const bool fast7 = !FUSED && nin > 0 && a.proj0.W <= 4 * bd;
+  int wflag = 0, pos_w = 0, walk_tot = 0;
+  uint32_t band_pre[4] = {0u, 0u, 0u, 0u}, band_tot[4] = {0u, 0u, 0u, 0u};
+  if (fast7) {
+    if (a.proj0.dK && live && spk && dcnt0 + 1 - dfl0 >= a.proj0.dK - 1) wflag = 1;
+    uint32_t sa = (uint32_t)(live ? spk : 0) | ((uint32_t)wflag << 16);
+    uint32_t sb = (uint32_t)__popc(fut0) | ((uint32_t)__popc(fut1) << 16);
+    uint32_t sc = (uint32_t)__popc(fut2) | ((uint32_t)__popc(fut3) << 16);
+    uint32_t tot[3];
+    block_scan3(sa, sb, sc, sh_scan3, tot);
+    pos_s = (int)(sa & 0xffffu);
+    pos_w = (int)(sa >> 16);
+    spk_tot = (int)(tot[0] & 0xffffu);
+    walk_tot = (int)(tot[0] >> 16);
+    band_pre[0] = sb & 0xffffu;
+    band_pre[1] = sb >> 16;
+    band_pre[2] = sc & 0xffffu;
+    band_pre[3] = sc >> 16;
+    band_tot[0] = tot[1] & 0xffffu;
+    band_tot[1] = tot[1] >> 16;
+    band_tot[2] = tot[2] & 0xffffu;
+    band_tot[3] = tot[2] >> 16;
+  } else {
     pos_s = block_scan_fast(live ? spk : 0, sh_scan, &spk_tot);
+  }
+  {
     if (live && spk) {
       sh_spk[pos_s] = j;
       sh_ext[pos_s] = fo0; /* the fan-out range of the first outgoing projection */
@@ -881,7 +955,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
   /* 7a. the first projection's ring slot of this exec step into LDS (from
    * the words read at the start; a slot of more than four words per thread
    * reads the rest here, before the stores) */
-  if (nin > 0) {
+  if (nin > 0 && !fast7) {
     const int W = a.proj0.W;
     const uint32_t* words = a.proj0.fut + ((int64_t)(ps.exec % GSBN_FUT) * a.H + h) * W;
     for (int w = j, i = 0; w < W; w += bd, ++i)
@@ -946,18 +1020,26 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
   if (nin > 0 && a.proj0.dK) {
     const gsbn_pop_proj& p0 = a.proj0;
     const int K = p0.dK;
-    int wflag = 0, wT = 0;
+    int wflag6 = 0, wT = 0;
     if (live && spk) {
       const int n = dcnt0;
       p0.dhist[(int64_t)idx * K + (n & (K - 1))] = ps.simstep;
       p0.dcnt[idx] = n + 1;
       if (n + 1 - dfl0 >= K - 1) {
         p0.dfl[idx] = n + 1;
-        wflag = 1;
+        wflag6 = 1;
         wT = n + 1;
       }
     }
-    if (spk_tot > 0) {
+    if (fast7) {
+      /* positions from the scan of step 5 */
+      if (wflag6) {
+        p0.walk_list[h * M + pos_w] = j;
+        p0.walk_T[h * M + pos_w] = wT;
+      }
+      if (j == 0) p0.walk_cnt[h] = walk_tot;
+    } else if (spk_tot > 0) {
+      const int wflag = wflag6;
       int wtot;
       const int pos = block_scan_fast(wflag, sh_scan, &wtot);
       if (wflag) {
@@ -1029,7 +1111,49 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
      * ends, or the fused caller synchronises before reading the lists */
     if (m + 1 < a.nproj_in) __syncthreads();
   };
-  if (nin > 0) active_rows(a.proj0, 0, std::true_type{});
+  if (fast7) {
+    /* the first projection's active rows from the ring words in registers
+     * (word j + i*bd of the slot in fut_i), placed by the scan of step 5:
+     * band i (words i*bd .. i*bd+bd-1) starts after the bands before it */
+    const gsbn_pop_proj& p = a.proj0;
+    const int C = p.C;
+    const int W = p.W;
+    uint32_t* words = p.fut + ((int64_t)(ps.exec % GSBN_FUT) * a.H + h) * W;
+    const uint32_t fw[4] = {fut0, fut1, fut2, fut3};
+    uint32_t base = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int w = j + i * bd;
+      uint32_t bits = fw[i];
+      if (bits) {
+        words[w] = 0u;
+        int pos = (int)(base + band_pre[i]);
+        while (bits) {
+          const int b = __ffs(bits) - 1;
+          bits &= bits - 1;
+          const int k = (w << 5) + b;
+          p.rows_list[(int64_t)h * C + pos] = k;
+          p.act_stamp[(int64_t)h * C + k] = ps.exec;
+          ++pos;
+        }
+      }
+      base += band_tot[i];
+    }
+    if (j == 0) {
+      const int tot = (int)base;
+      p.rows_cnt[h] = tot;
+      p.pend_cnt[h] = tot;
+      if (a.work) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.work[0]),
+                  (unsigned long long)tot * (unsigned long long)M);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.work[1]),
+                  (unsigned long long)spk_tot * (unsigned long long)C);
+      }
+    }
+    if (nin > 1) __syncthreads();
+  } else if (nin > 0) {
+    active_rows(a.proj0, 0, std::true_type{});
+  }
   for (int m = 1; m < nin; ++m) active_rows(a.projs[m], m, std::false_type{});
 }
 
