@@ -44,6 +44,7 @@ static_assert(sizeof(gsbn_syn16_rec) == 16 && sizeof(gsbn_syn16_t) == 16,
 #define POP_MAX_THREADS 1024
 #define RC_THREADS 256
 #define RC_COLS 4 /* column slots per thread and pass */
+#define GSBN_MAX_DEFER 64 /* largest ring of deferred column kicks per unit */
 #define ALL_THREADS 256
 #define ALL_ROWS 16
 
@@ -1331,12 +1332,35 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
   if (!FUSED && a.dK) {
     /* deferred column updates: walk the listed units' columns, applying
      * their pending kicks (up to this step's) to every slot with ii >= 0
-     * that this step's row pass does not own */
+     * that this step's row pass does not own.  The unit's ring entries (in
+     * spike order) and the coefficients of the gaps between consecutive
+     * entries are the same for every slot of the column: they are read once
+     * per walk into LDS, so a slot loads only the coefficients of its first
+     * pending kick (the gap from its own tij). */
+    __shared__ int sh_wring[GSBN_MAX_DEFER];
+    __shared__ float sh_wgc[GSBN_MAX_DEFER * 8];
     for (int s_i = 0; s_i < nspk; ++s_i) {
       const int j = s_i == 0 ? j_pre : a.spk_list[h * M + s_i];
       const int uj = h * M + j;
       const int32_t* ring = a.dhist + (int64_t)uj * a.dK;
       const int T = a.walk_T[h * M + s_i];
+      const int lo = T - a.dK + 1 > 0 ? T - a.dK + 1 : 0;
+      const int cnt = T - lo;
+      __syncthreads(); /* the previous walk's LDS */
+      if (tid < cnt) sh_wring[tid] = ring[(lo + tid) & (a.dK - 1)];
+      __syncthreads();
+      if (tid + 1 < cnt) {
+        const gsbn_syn_coef_t c = load_coef(q.table, q.table_len, sh_wring[tid + 1] - sh_wring[tid]);
+        float* g = sh_wgc + tid * 8;
+        g[0] = c.ezi;
+        g[1] = c.ezj;
+        g[2] = c.ee;
+        g[3] = c.ep;
+        g[4] = c.cez;
+        g[5] = c.cpe;
+        g[6] = c.cpz;
+      }
+      __syncthreads();
       for (int kb = k0 + tid; kb < k1; kb += U * NT) {
         bool ok[U];
         int64_t sidx[U];
@@ -1349,10 +1373,40 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
           ok[u] = (k < k1) & (a.ii[r] >= 0) & (a.act_stamp[r] != q.exec);
           v[u] = io.load_traces(sidx[u]);
         }
+        /* first pending kick of each slot (entries above its tij; all
+         * entries are at most this step) and its coefficients */
+        int f[U];
+        gsbn_syn_coef_t c0[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (ok[u] && replay_kicks<PREC>(io, v[u], ring, T, a.dK, t, q.table, q.table_len, a.kftj))
-            io.store_traces(sidx[u], v[u]);
+        for (int u = 0; u < U; ++u) {
+          f[u] = cnt;
+          for (int n = cnt - 1; n >= 0 && sh_wring[n] > v[u].tij; --n) f[u] = n;
+          if (ok[u] && f[u] < cnt) c0[u] = load_coef(q.table, q.table_len, sh_wring[f[u]] - v[u].tij);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!ok[u] || f[u] >= cnt) continue;
+          for (int n = f[u]; n < cnt; ++n) {
+            gsbn_syn_coef_t c;
+            if (n == f[u]) {
+              c = c0[u];
+            } else {
+              const float* g = sh_wgc + (n - 1) * 8;
+              c.ezi = g[0];
+              c.ezj = g[1];
+              c.ee = g[2];
+              c.ep = g[3];
+              c.cez = g[4];
+              c.cpe = g[5];
+              c.cpz = g[6];
+            }
+            gsbn_syn_catchup_k(c, &v[u].pij, &v[u].eij, &v[u].zi2, &v[u].zj2);
+            v[u].zj2 = v[u].zj2 + a.kftj;
+            v[u].tij = sh_wring[n];
+            io.round_traces(v[u]);
+          }
+          io.store_traces(sidx[u], v[u]);
+        }
       }
     }
     return;
@@ -1904,7 +1958,7 @@ static bool trace_form_ok(const A& a) {
 /* deferred column updates: dK 0 (off) or a power of two in [2, 64] with its
  * arrays */
 static bool deferred_ok(int32_t dK, const void* ii, const void* dhist, const void* dcnt) {
-  return dK == 0 || (dK >= 2 && dK <= 64 && (dK & (dK - 1)) == 0 && ii && dhist && dcnt);
+  return dK == 0 || (dK >= 2 && dK <= GSBN_MAX_DEFER && (dK & (dK - 1)) == 0 && ii && dhist && dcnt);
 }
 
 static size_t pop_lds(int threads, int64_t max_slots, bool fused) {
