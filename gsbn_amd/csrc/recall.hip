@@ -207,6 +207,113 @@ __global__ void __launch_bounds__(RE_THREADS_MAX) recall_step_kernel(ReArgs a, i
   }
 }
 
+/* The population step of the mfma mode with one wave per (HCU, cue) and
+ * four pairs per workgroup: lane l takes units l, l + 64, ... (UPL per
+ * lane), so no workgroup barrier is needed and a hypercolumn of M <= 64 * UPL
+ * units occupies one wave instead of ceil(M / 64).  Same operations per unit
+ * as recall_step_kernel: the WTA maximum combines the 64-unit ranges left to
+ * right with the sequential comparison, and the normaliser is summed in
+ * ascending unit order by lane 0 from the wave's LDS row. */
+template <int UPL>
+__global__ void __launch_bounds__(256) recall_pop_wave_kernel(ReArgs a, int t) {
+  __shared__ __attribute__((aligned(16))) float sh_act[4][UPL * 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t pair = (int64_t)blockIdx.x * 4 + wave; /* b * H + h */
+  if (pair >= (int64_t)a.B * a.H) return;              /* uniform per wave */
+  const int b = (int)(pair / a.H), h = (int)(pair - (int64_t)b * a.H);
+  const int M = a.M, N = a.N;
+  const int cl = a.clamp[pair];
+  const float wm = a.wmask[pair];
+  float* const row = sh_act[wave];
+  float dsup[UPL], u[UPL];
+  int cnt[UPL];
+  /* support, dsup (Pop.cpp:229-261) */
+#pragma unroll
+  for (int i = 0; i < UPL; ++i) {
+    const int j = lane + 64 * i;
+    dsup[i] = 0.0f;
+    u[i] = 0.0f;
+    cnt[i] = 0;
+    if (j < M) {
+      const int n = h * M + j;
+      const int64_t bn = (int64_t)b * N + n;
+      const float E = a.epsc[bn];
+      const float bjv = a.bj[n];
+      const float d0 = a.dsup[bn];
+      if (t > a.count_from) cnt[i] = a.count[bn];
+      float z;
+      gsbn_draw_unit(a.seed + (uint64_t)b, 0u, (uint32_t)t, (uint32_t)n, &u[i], &z);
+      const float xi = a.snoise * z;
+      float wsup = 0.0f;
+      wsup += bjv + E;
+      const float lginp = cl == j ? a.lg_on : a.lg_off;
+      float sup = a.lgbias + a.igain * lginp + xi;
+      sup += (a.wgain * wm) * wsup;
+      dsup[i] = d0 + (sup - d0) * a.taumdt;
+      a.dsup[bn] = dsup[i];
+    }
+  }
+  /* WTA maximum: each 64-unit range by shuffles, the ranges left to right */
+  float maxdsup = 0.0f;
+#pragma unroll
+  for (int i = 0; i < UPL; ++i) {
+    float mv = lane + 64 * i < M ? dsup[i] : -INFINITY;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const float o = __shfl_down(mv, off, 64);
+      if ((lane & (2 * off - 1)) == 0 && o > mv) mv = o;
+    }
+    mv = __shfl(mv, 0, 64);
+    if (i == 0) {
+      maxdsup = mv;
+    } else if (64 * i < M && mv > maxdsup) {
+      maxdsup = mv;
+    }
+  }
+  /* soft-max (Pop.cpp:280-302) */
+  const float maxact = gsbn_expf(a.wtagain * maxdsup);
+  float act[UPL];
+#pragma unroll
+  for (int i = 0; i < UPL; ++i) {
+    act[i] = 0.0f;
+    if (lane + 64 * i < M) {
+      act[i] = gsbn_expf(a.wtagain * (dsup[i] - maxdsup));
+      if (maxact < 1.0f) act[i] *= maxact;
+      row[lane + 64 * i] = act[i];
+    }
+  }
+  float vsum = 0.0f;
+  if (lane == 0) {
+    const float4* v4 = reinterpret_cast<const float4*>(row);
+    int m = 0;
+#pragma unroll 5
+    for (; m + 4 <= M; m += 4) {
+      const float4 x = v4[m >> 2];
+      vsum += x.x;
+      vsum += x.y;
+      vsum += x.z;
+      vsum += x.w;
+    }
+    for (; m < M; ++m) vsum += row[m];
+  }
+  vsum = __shfl(vsum, 0, 64);
+  /* spikes (Pop.cpp:305-323) */
+  const int slot = t % RE_RING;
+#pragma unroll
+  for (int i = 0; i < UPL; ++i) {
+    const int j = lane + 64 * i;
+    if (j >= M) continue;
+    float ac = act[i];
+    if (vsum > 1.0f) ac /= vsum;
+    const int spk = u[i] < ac * a.maxfqdt ? 1 : 0;
+    const int n = h * M + j;
+    const int64_t bn = (int64_t)b * N + n;
+    if (t > a.count_from) a.count[bn] = cnt[i] + spk;
+    if (a.raster) a.raster[bn] = (int8_t)spk;
+    a.S[((int64_t)slot * a.Bp + b) * a.Kp + n] = spk ? (uint16_t)0x3F80u : (uint16_t)0u;
+  }
+}
+
 /* E[b][n] = beta E[b][n] + sum_k S[b][k] WT[n][k]: bf16 inputs, float
  * accumulation on v_mfma_f32_32x32x16_bf16.  S and WT have Kp columns (Kp a
  * multiple of GM_BK, zero padded); S has ceil(B / GM_BM) * GM_BM rows and WT
@@ -595,8 +702,21 @@ int gsbn_recall_run(gsbn_recall* r, const int16_t* clamp, const float* wmask, in
   const size_t lds = d.mode == GSBN_RECALL_EXACT ? (size_t)r->W * 4 : 0;
   const float beta = 1 - d.kepsc;
   double gemm_ms = 0.0;
+  /* the mfma mode's population step: one wave per (HCU, cue) up to 256 units */
+  const int upl = (M + 63) / 64;
+  const bool wave_step = d.mode == GSBN_RECALL_MFMA && upl <= 4;
+  const unsigned wave_grid = (unsigned)(((int64_t)B * H + 3) / 4);
   for (int t = 1; t <= steps; ++t) {
-    hipLaunchKernelGGL(recall_step_kernel, dim3(H, B), dim3(threads), lds, s, a, t);
+    if (wave_step) {
+      if (upl == 1)
+        hipLaunchKernelGGL(recall_pop_wave_kernel<1>, dim3(wave_grid), dim3(256), 0, s, a, t);
+      else if (upl == 2)
+        hipLaunchKernelGGL(recall_pop_wave_kernel<2>, dim3(wave_grid), dim3(256), 0, s, a, t);
+      else
+        hipLaunchKernelGGL(recall_pop_wave_kernel<4>, dim3(wave_grid), dim3(256), 0, s, a, t);
+    } else {
+      hipLaunchKernelGGL(recall_step_kernel, dim3(H, B), dim3(threads), lds, s, a, t);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return fail((int)e, "recall_step_kernel");
     if (d.mode == GSBN_RECALL_MFMA) {
       const int src = t - r->delay - 2;

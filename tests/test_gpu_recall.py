@@ -230,3 +230,37 @@ def test_recall_8192_cues_exact_sampled_against_oracle(trained100):
         net.close()
     assert spiking > 0
     del ras
+
+
+@pytest.mark.parametrize("M", [40, 100, 200])
+def test_recall_mfma_population_step_matches_exact(M):
+    # with integer weights every support sum is exact in float and in the
+    # bfloat16 matrix product, and kepsc = 0 keeps the decay exact, so the
+    # mfma mode's support equals the exact mode's; its one-wave-per-(HCU,
+    # cue) population step (1, 2 and 4 units per lane) must then give the
+    # exact mode's spikes, winners and counts bit for bit.  37 cues: the last
+    # workgroup of four (HCU, cue) pairs is partly empty.
+    H, B, T = 6, 37, 30
+    C = (H - 1) * M
+    rng = np.random.default_rng(M)
+    ii = np.concatenate([np.array([u for u in range(H * M) if u // M != h], np.int32)
+                         for h in range(H)])
+    di = np.ones(H * C, np.int32)
+    wij = rng.integers(-2, 3, size=H * C * M).astype(np.float32)
+    bj = rng.uniform(-1.0, 0.0, size=H * M).astype(np.float32)
+    params = dict(taumdt=0.1, wtagain=4.0, maxfqdt=0.1, igain=1.0, wgain=0.05, lgbias=0.0,
+                  snoise=0.3, kepsc=0.0, lg_on=float(np.log(np.float32(1.001))),
+                  lg_off=float(np.log(np.float32(0.001))))
+    clamp = np.where(rng.random((B, H)) < 0.5, rng.integers(0, M, (B, H)), -1).astype(np.int16)
+    wmask = rng.random((B, H)).astype(np.float32)
+    out = {}
+    for mode in (R.EXACT, R.MFMA):
+        rc = R.Recall(H, M, C, ii, di, wij, bj, params, B, mode, seed=5)
+        out[mode] = rc.run(clamp, wmask, T, count_from=0, raster=True)
+        rc.close()
+    (ie, ce, re_, _), (im, cm, rm, _) = out[R.EXACT], out[R.MFMA]
+    spikes = int(re_.sum())
+    assert 0 < spikes < re_.size
+    for t in range(T):
+        assert np.array_equal(re_[t], rm[t]), "step %d differs" % (t + 1)
+    assert np.array_equal(ie, im) and np.array_equal(ce, cm)
