@@ -34,6 +34,7 @@
 #include "detmath.h"
 #include "gsbn_core.h"
 #include "philox.h"
+#include "wave.h"
 
 static_assert(sizeof(gsbn_syn_rec) == 32, "synapse record must be 32 bytes");
 static_assert(sizeof(gsbn_syn_t) == sizeof(gsbn_syn_rec), "record layouts differ");
@@ -125,13 +126,9 @@ unsigned loop_grid(K kernel, int threads, size_t lds, int64_t n) {
  * sh needs blockDim/64 + 1 ints */
 __device__ int block_excl_scan(int v, int* sh, int* total) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) sh[wid] = x;
+  const int x = (int)wave_incl_scan((uint32_t)v);
+  (void)lane;
+  if ((threadIdx.x & 63) == 63) sh[wid] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
     int acc = 0;
@@ -152,12 +149,7 @@ __device__ int block_excl_scan(int v, int* sh, int* total) {
 /* the same with a wave shuffle scan and two barriers (no serial section) */
 __device__ __forceinline__ int block_scan_fast(int v, int* sh, int* total) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
+  const int x = (int)wave_incl_scan((uint32_t)v);
   if (lane == 63) sh[wid] = x;
   __syncthreads();
   int base = 0, tot = 0;
@@ -178,18 +170,7 @@ __device__ __forceinline__ int block_scan_fast(int v, int* sh, int* total) {
 __device__ __forceinline__ void block_scan3(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t* sh,
                                             uint32_t* tot) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  uint32_t x = a, y = b, z = c;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t x1 = __shfl_up(x, o, 64);
-    const uint32_t y1 = __shfl_up(y, o, 64);
-    const uint32_t z1 = __shfl_up(z, o, 64);
-    if (lane >= o) {
-      x += x1;
-      y += y1;
-      z += z1;
-    }
-  }
+  const uint32_t x = wave_incl_scan(a), y = wave_incl_scan(b), z = wave_incl_scan(c);
   if (lane == 63) {
     sh[3 * wid] = x;
     sh[3 * wid + 1] = y;
@@ -832,15 +813,11 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
   /* 3. WTA maximum (update_sup_kernel_2_cpu, Pop.cpp:270-279): the
    * reference's sequential `if (d > max) max = d` keeps the first of equal
    * maxima; combining contiguous index ranges left to right with the same
-   * comparison gives the same value (also for +-0 ties and NaN), here
-   * within each wave by shuffles, then across waves in order */
+   * comparison gives the same value (+-0 ties included, for NaN-free
+   * dsup), here within each wave (wave_max_left), then across waves in
+   * order */
   {
-    float mv = live ? dsup : -INFINITY;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const float o = __shfl_down(mv, off, 64);
-      if ((j & (2 * off - 1)) == 0 && o > mv) mv = o;
-    }
+    const float mv = wave_max_left(live ? dsup : -INFINITY);
     if ((j & 63) == 0) sh_wmax[j >> 6] = mv;
   }
   __syncthreads();

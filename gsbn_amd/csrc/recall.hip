@@ -40,6 +40,7 @@
 #include "detmath.h"
 #include "gsbn_core.h"
 #include "philox.h"
+#include "wave.h"
 
 #define RE_THREADS_MAX 1024
 #define RE_RING 64 /* spike-history slots; latency di + 2 <= 33 */
@@ -110,12 +111,7 @@ __global__ void __launch_bounds__(RE_THREADS_MAX) recall_step_kernel(ReArgs a, i
   }
   /* WTA maximum: first of equal maxima, as the sequential reference */
   {
-    float mv = live ? dsup : -INFINITY;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const float o = __shfl_down(mv, off, 64);
-      if ((j & (2 * off - 1)) == 0 && o > mv) mv = o;
-    }
+    const float mv = wave_max_left(live ? dsup : -INFINITY);
     if ((j & 63) == 0) sh_f[j >> 6] = mv;
   }
   __syncthreads();
@@ -257,13 +253,7 @@ __global__ void __launch_bounds__(256) recall_pop_wave_kernel(ReArgs a, int t) {
   float maxdsup = 0.0f;
 #pragma unroll
   for (int i = 0; i < UPL; ++i) {
-    float mv = lane + 64 * i < M ? dsup[i] : -INFINITY;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const float o = __shfl_down(mv, off, 64);
-      if ((lane & (2 * off - 1)) == 0 && o > mv) mv = o;
-    }
-    mv = __shfl(mv, 0, 64);
+    const float mv = wave_max_left(lane + 64 * i < M ? dsup[i] : -INFINITY);
     if (i == 0) {
       maxdsup = mv;
     } else if (64 * i < M && mv > maxdsup) {
@@ -296,7 +286,7 @@ __global__ void __launch_bounds__(256) recall_pop_wave_kernel(ReArgs a, int t) {
     }
     for (; m < M; ++m) vsum += row[m];
   }
-  vsum = __shfl(vsum, 0, 64);
+  vsum = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vsum)));
   /* spikes (Pop.cpp:305-323) */
   const int slot = t % RE_RING;
 #pragma unroll
