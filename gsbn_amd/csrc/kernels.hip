@@ -638,7 +638,7 @@ __device__ __forceinline__ PopStep pop_step(const gsbn_pop_args& a, const gsbn_s
  * contributions are read before their count is known), and every global
  * store of the update is issued after the last load: loads and stores share
  * one in-order counter, so a load issued behind stores would wait for them. */
-template <bool FUSED, int CPREC = -1>
+template <bool FUSED, int CPREC = -1, int NPRE = 4, bool SPEC = true>
 __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& ps, const int h,
                                          FusedLds* f = nullptr,
                                          const gsbn_rowcol_args* rb = nullptr,
@@ -668,10 +668,11 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
 
   /* 0. loads that do not depend on this step's results */
   float dsup_old = 0.0f, lg = 0.0f, e0 = 0.0f, bj0 = 0.0f, pj0 = 0.0f, ej0 = 0.0f, zj0 = 0.0f;
-  /* pending contributions read before their count is known (speculative
-   * reads cost traffic when few rows are pending, as in batches; the rest
-   * is read eight rows per round trip below) */
-  constexpr int NPRE = 4;
+  /* pending contributions read before their count is known (NPRE of them:
+   * four for a single network, whose busy steps have several rows pending
+   * per hypercolumn; one for batches, whose hypercolumns mostly have none,
+   * where more speculative reads cost more than the round trip they save;
+   * the rest is read eight rows per round trip below) */
   float cpre[NPRE];
 #pragma unroll
   for (int q = 0; q < NPRE; ++q) cpre[q] = 0.0f;
@@ -686,9 +687,11 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
     counter_old = a.counter[idx];
   }
   /* fan-out range of this unit in the first outgoing projection (the queue
-   * push of step 5b) */
+   * push of step 5b).  SPEC: read now, for every unit; otherwise (batches,
+   * where few units spike in a step) read by the spiking units once their
+   * spike is known, as is their deferred ring state */
   int fo0 = 0, fo1 = 0;
-  if (a.nproj_out > 0 && live) {
+  if (SPEC && a.nproj_out > 0 && live) {
     fo0 = a.out0.fwd_off[idx];
     fo1 = a.out0.fwd_off[idx + 1];
   }
@@ -708,7 +711,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
 #pragma unroll
       for (int q = 0; q < NPRE; ++q)
         if (p0.C > q) cpre[q] = c[(int64_t)q * M];
-      if (p0.dK) {
+      if (SPEC && p0.dK) {
         dcnt0 = p0.dcnt[idx];
         dfl0 = p0.dfl[idx];
       }
@@ -876,6 +879,17 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
     if (live) spk = sh_ext[j];
   }
   if (FUSED && live) sh_sj[j] = spk;
+
+  if (!SPEC && live && spk) {
+    if (a.nproj_out > 0) {
+      fo0 = a.out0.fwd_off[idx];
+      fo1 = a.out0.fwd_off[idx + 1];
+    }
+    if (nin > 0 && a.proj0.dK) {
+      dcnt0 = a.proj0.dcnt[idx];
+      dfl0 = a.proj0.dfl[idx];
+    }
+  }
 
   /* 5. spiking units of this hypercolumn, ascending (.ssj, Proj.cpp:632-643).
    * Without the fused caller, and when the first projection's ring slot is
@@ -1159,7 +1173,7 @@ __global__ void __launch_bounds__(POP_MAX_THREADS)
     if (w != (int)blockIdx.x) __syncthreads(); /* LDS of the previous hypercolumn */
     const gsbn_pop_args& a = arr[wg_pop[w]];
     touch_lines<sizeof(gsbn_pop_args)>(&a);
-    pop_body<false>(a, pop_step(a, st), wg_hcu[w]);
+    pop_body<false, -1, 1, false>(a, pop_step(a, st), wg_hcu[w]);
   }
 }
 
