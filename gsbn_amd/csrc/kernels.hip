@@ -1165,8 +1165,11 @@ __device__ __forceinline__ void apply_step(gsbn_pop_args& a, const gsbn_step_arg
 }
 
 /* Workgroup b updates hypercolumns b, b + gridDim.x, ... of the map (the
- * launch gives one hypercolumn to each workgroup, see loop_grid). */
-__global__ void __launch_bounds__(POP_MAX_THREADS)
+ * launch gives one hypercolumn to each workgroup, see loop_grid).  The
+ * launch is throughput-bound over many short latency chains: six waves per
+ * SIMD (80 registers, no spills) keep more of them in flight than the five
+ * the kernel's natural register count allows. */
+__global__ void __launch_bounds__(POP_MAX_THREADS, 6)
     pop_kernel_batch(const gsbn_pop_args* __restrict__ arr, const int32_t* __restrict__ wg_pop,
                      const int32_t* __restrict__ wg_hcu, int nwg, gsbn_step_args st) {
   for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
