@@ -687,9 +687,10 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
     counter_old = a.counter[idx];
   }
   /* fan-out range of this unit in the first outgoing projection (the queue
-   * push of step 5b).  SPEC: read now, for every unit; otherwise (batches,
-   * where few units spike in a step) read by the spiking units once their
-   * spike is known, as is their deferred ring state */
+   * push of step 5b).  SPEC: read now, for every unit; otherwise read by the
+   * spiking units once their spike is known, as is their deferred ring
+   * state (measured faster for the single and the batched launches: few
+   * units spike in a step) */
   int fo0 = 0, fo1 = 0;
   if (SPEC && a.nproj_out > 0 && live) {
     fo0 = a.out0.fwd_off[idx];
@@ -1151,7 +1152,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
 
 __global__ void __launch_bounds__(POP_MAX_THREADS) pop_kernel(gsbn_pop_args a) {
   touch_kernarg_lines<sizeof(gsbn_pop_args)>();
-  pop_body<false>(a, pop_step(a), blockIdx.x);
+  pop_body<false, -1, 4, false>(a, pop_step(a), blockIdx.x);
 }
 
 /* per-step fields shared by every population / projection of a batch */
