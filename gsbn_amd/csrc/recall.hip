@@ -44,6 +44,7 @@
 
 #define RE_THREADS_MAX 1024
 #define RE_RING 64 /* spike-history slots; latency di + 2 <= 33 */
+#define RE_ROWS_MAX 2048 /* active rows compacted into LDS per (HCU, cue) and step */
 
 /* GEMM tiling: 256 x 256 output tile per workgroup of 16 waves (64 x 64
  * each, 2 x 2 blocks of 32 x 32), K in steps of 64, LDS rows padded by 8 bf16 */
@@ -79,6 +80,24 @@ struct ReArgs {
 };
 
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
+
+/* exclusive prefix sum across the workgroup (blockDim a multiple of 64,
+ * sh: blockDim / 64 ints), with the total */
+__device__ __forceinline__ int block_scan_fast_re(int v, int* sh, int* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int x = (int)wave_incl_scan((uint32_t)v);
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+  for (int w = 0; w < nw; ++w) {
+    const int t = sh[w];
+    if (w < wid) base += t;
+    tot += t;
+  }
+  *total = tot;
+  __syncthreads();
+  return base + x - v;
+}
 
 /* one workgroup per (HCU, cue): the population step and, in the exact mode,
  * the support update of that HCU's units */
@@ -188,15 +207,51 @@ __global__ void __launch_bounds__(RE_THREADS_MAX) recall_step_kernel(ReArgs a, i
     }
   }
   __syncthreads();
+  /* the active rows in ascending order, compacted into LDS (after the bitmap
+   * in the dynamic LDS), so each thread can have eight weight loads in
+   * flight instead of one per row; more than RE_ROWS_MAX rows (not reached
+   * by recall, whose active rows are the few spiking sources) fall back to
+   * walking the bitmap */
+  int* const sh_rows = reinterpret_cast<int*>(sh_bits + Wd);
+  int nrows;
+  {
+    const int per = (Wd + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int w0 = j * per;
+    const int w1 = w0 + per < Wd ? w0 + per : Wd;
+    int c = 0;
+    for (int w = w0; w < w1; ++w) c += __popc(sh_bits[w]);
+    int pos = block_scan_fast_re(c, sh_scan, &nrows);
+    if (nrows <= RE_ROWS_MAX) {
+      for (int w = w0; w < w1; ++w) {
+        uint32_t bits = sh_bits[w];
+        while (bits) {
+          sh_rows[pos++] = (w << 5) + __ffs(bits) - 1;
+          bits &= bits - 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
   if (live) {
     float e = E * (1 - a.kepsc);
     const float* wrow = a.wij + (int64_t)h * C * M + j;
-    for (int w = 0; w < Wd; ++w) {
-      uint32_t bits = sh_bits[w];
-      while (bits) {
-        const int k = (w << 5) + __ffs(bits) - 1;
-        bits &= bits - 1;
-        e = e + wrow[(int64_t)k * M];
+    if (nrows <= RE_ROWS_MAX) {
+      for (int r = 0; r < nrows; r += 8) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = r + q < nrows ? wrow[(int64_t)sh_rows[r + q] * M] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (r + q < nrows) e = e + v[q];
+      }
+    } else {
+      for (int w = 0; w < Wd; ++w) {
+        uint32_t bits = sh_bits[w];
+        while (bits) {
+          const int k = (w << 5) + __ffs(bits) - 1;
+          bits &= bits - 1;
+          e = e + wrow[(int64_t)k * M];
+        }
       }
     }
     a.epsc[bn] = e;
@@ -689,7 +744,7 @@ int gsbn_recall_run(gsbn_recall* r, const int16_t* clamp, const float* wmask, in
   a.wmask = r->wmask.p;
   a.raster = raster ? r->raster.p : nullptr;
   const int threads = ((M + 63) / 64) * 64;
-  const size_t lds = d.mode == GSBN_RECALL_EXACT ? (size_t)r->W * 4 : 0;
+  const size_t lds = d.mode == GSBN_RECALL_EXACT ? ((size_t)r->W + RE_ROWS_MAX) * 4 : 0;
   const float beta = 1 - d.kepsc;
   double gemm_ms = 0.0;
   /* the mfma mode's population step: one wave per (HCU, cue) up to 256 units */
