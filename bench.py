@@ -589,7 +589,7 @@ def main():
                          "per HCU), fused, chained, or wave (batches: one wave per HCU)")
     ap.add_argument("--deferred", type=int, default=None, metavar="K",
                     help="deferred column updates with rings of K spike steps per unit "
-                         "(ProcUpdLazy deferred-columns; 0: eager; default: the library's, 8)")
+                         "(ProcUpdLazy deferred-columns; 0: eager; default: the library's, 8 for one network, 16 for a batch)")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--cpu-arith", default="ref32", choices=("ref32", "table"))
     ap.add_argument("--step-times", type=int, default=0,

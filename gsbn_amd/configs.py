@@ -147,7 +147,7 @@ class NetSpec:
     rowcol_wave: int = 0
     # deferred_columns K (a power of two, 2..64): column updates deferred
     # through rings of K spike steps per postsynaptic unit; 0: eager; None:
-    # ProcUpdLazy's default (8 where it applies)
+    # ProcUpdLazy's default (where it applies: 8 for one network, 16 for a batch)
     deferred_columns: Optional[int] = None
 
     @property

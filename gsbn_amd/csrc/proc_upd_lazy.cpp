@@ -1277,9 +1277,14 @@ class ProcUpdLazy : public ProcedureBase {
    * hypercolumn (plan -20) */
   int _wave_fixed = 0;
   /* deferred-columns K: column updates deferred through rings of K spike
-   * steps per unit (0: eager); see Proj::dK.  Default 8 where it applies
-   * (same results, a fraction of the column traffic). */
-  static constexpr int kDeferDefault = 8;
+   * steps per unit (0: eager); see Proj::dK.  Same results for every K, a
+   * fraction of the column traffic.  Default where it applies: 8 for one
+   * network (its row update replays a ring of 8 from registers), 16 for a
+   * batch, whose one-wave-per-hypercolumn row/column launch is dominated by
+   * column walks at 8 (measured on 512 networks: 50.8 us at 8, 38.9 us at
+   * 16, 39.1 us at 32 and 64). */
+  static constexpr int kDeferDefault = 8, kDeferBatch = 16;
+  int _defer_arg = -1; /* the argument, -1 when not given */
   int _defer_k = kDeferDefault;
   void end_deferral();
   bool _tuned = false;
@@ -1421,8 +1426,9 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   /* rowcol-graded g (1-3): always the graded split g of a batch */
   par.argi("rowcol-graded", _graded_fixed);
   par.argi("rowcol-wave", _wave_fixed);
-  par.argi("deferred-columns", _defer_k);
-  if (_defer_k < 0 || _defer_k == 1 || _defer_k > 64 || (_defer_k & (_defer_k - 1)))
+  par.argi("deferred-columns", _defer_arg);
+  if (_defer_arg != -1 &&
+      (_defer_arg < 0 || _defer_arg == 1 || _defer_arg > 64 || (_defer_arg & (_defer_arg - 1))))
     throw Error(-2, "ProcUpdLazy: deferred-columns must be 0 or a power of two in [2, 64]");
   if (_graded_fixed < 0 || _graded_fixed > 3)
     throw Error(-2, "ProcUpdLazy: rowcol-graded must be in [0, 3]");
@@ -1664,6 +1670,7 @@ void ProcUpdLazy::update_gpu() {
     }
     /* deferred column updates: every population with exactly one incoming
      * projection (the population kernel keeps the first projection's rings) */
+    _defer_k = _defer_arg >= 0 ? _defer_arg : _batch ? kDeferBatch : kDeferDefault;
     if (_defer_k) {
       bool ok = !_list_proj.empty();
       for (auto* p : _list_pop)

@@ -29,7 +29,8 @@ def _run(spec, tmp_path, max_steps=None, check_every=0):
     assert div is None, "diverged at %s" % (div,)
     bad = helpers.compare_state(gpu, ora, spec)
     assert not bad, bad
-    assert gpu.glob("deferred-columns") == spec.deferred_columns
+    if spec.deferred_columns is not None:
+        assert gpu.glob("deferred-columns") == spec.deferred_columns
     return gpu, ora
 
 
@@ -91,13 +92,15 @@ def test_reference_sparse_delay_map(tmp_path):
     gpu.close()
 
 
-@pytest.mark.parametrize("wave,K", [(0, 4), (1, 4), (0, 0)])
+@pytest.mark.parametrize("wave,K", [(0, 4), (1, 4), (0, 0), (1, 8), (1, None), (0, None)])
 def test_batched_networks(tmp_path, wave, K):
+    # K None: the library's default for a batch, rings of 16
     spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=(3, 140), n_pat=4, train=40,
                                 quiet=5, recall=30, reset=20)
     spec.deferred_columns = K
     spec.rowcol_wave = wave
     gpu, ora = _run(spec, tmp_path)
+    assert gpu.glob("deferred-columns") == (16 if K is None else K)
     c, t, flags = gpu.check()
     assert (c, t) == (ora.check.correct, ora.check.total)
     gpu.close()
@@ -202,7 +205,8 @@ def test_fused_step_runs_eager(tmp_path):
 
 
 def test_default_is_deferred(tmp_path):
-    # without the argument ProcUpdLazy defers with rings of 8; 0 asks for eager
+    # without the argument ProcUpdLazy defers with rings of 8 (16 for a batch
+    # of networks, test_batched_networks); 0 asks for eager
     spec = configs.config0()
     gpu, ora, div, steps = helpers.run_lockstep(spec, tmp_path, max_steps=30)
     assert div is None and gpu.glob("deferred-columns") == 8
