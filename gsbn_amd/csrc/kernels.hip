@@ -846,21 +846,7 @@ __device__ __forceinline__ void pop_body(const gsbn_pop_args& a, const PopStep& 
     sh_f[j] = act;
   }
   __syncthreads();
-  if (j == 0) {
-    float vsum = 0.0f;
-    const float4* v4 = reinterpret_cast<const float4*>(sh_f);
-    int m = 0;
-#pragma unroll 5
-    for (; m + 4 <= M; m += 4) {
-      const float4 x = v4[m >> 2];
-      vsum += x.x;
-      vsum += x.y;
-      vsum += x.z;
-      vsum += x.w;
-    }
-    for (; m < M; ++m) vsum += sh_f[m];
-    sh_vsum = vsum;
-  }
+  if (j == 0) sh_vsum = ordered_sum_lds(sh_f, M);
   __syncthreads();
   const float vsum = sh_vsum;
   if (live && vsum > 1.0f) act /= vsum;

@@ -103,7 +103,7 @@ __device__ __forceinline__ int block_scan_fast_re(int v, int* sh, int* total) {
  * the support update of that HCU's units */
 __global__ void __launch_bounds__(RE_THREADS_MAX) recall_step_kernel(ReArgs a, int t) {
   extern __shared__ uint32_t sh_bits[];
-  __shared__ float sh_f[RE_THREADS_MAX];
+  __shared__ __attribute__((aligned(16))) float sh_f[RE_THREADS_MAX];
   __shared__ int sh_scan[RE_THREADS_MAX / 64 + 1];
   __shared__ float sh_vsum;
   const int h = blockIdx.x, b = blockIdx.y, j = threadIdx.x;
@@ -149,11 +149,7 @@ __global__ void __launch_bounds__(RE_THREADS_MAX) recall_step_kernel(ReArgs a, i
     sh_f[j] = act;
   }
   __syncthreads();
-  if (j == 0) {
-    float vsum = 0.0f;
-    for (int m = 0; m < M; ++m) vsum += sh_f[m];
-    sh_vsum = vsum;
-  }
+  if (j == 0) sh_vsum = ordered_sum_lds(sh_f, M);
   __syncthreads();
   if (live && sh_vsum > 1.0f) act /= sh_vsum;
   /* spikes (Pop.cpp:305-323) */
@@ -328,19 +324,7 @@ __global__ void __launch_bounds__(256) recall_pop_wave_kernel(ReArgs a, int t) {
     }
   }
   float vsum = 0.0f;
-  if (lane == 0) {
-    const float4* v4 = reinterpret_cast<const float4*>(row);
-    int m = 0;
-#pragma unroll 5
-    for (; m + 4 <= M; m += 4) {
-      const float4 x = v4[m >> 2];
-      vsum += x.x;
-      vsum += x.y;
-      vsum += x.z;
-      vsum += x.w;
-    }
-    for (; m < M; ++m) vsum += row[m];
-  }
+  if (lane == 0) vsum = ordered_sum_lds(row, M);
   vsum = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vsum)));
   /* spikes (Pop.cpp:305-323) */
   const int slot = t % RE_RING;
