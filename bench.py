@@ -513,6 +513,27 @@ def batched(steps: int, warmup: int, form: str = "fp32") -> dict:
             "dominant_traffic": traffic, "dominant_traffic_ratio": ratio}
 
 
+def vendor_gemm_tflops(m: int, n: int, k: int, reps: int = 10) -> float:
+    """The vendor library's rate on the support product's shape: torch.matmul
+    (hipBLASLt) of bf16 (m x k) and (k x n), bf16 out, no fused decay, timed
+    with HIP events; a scale for the hand-written product, not a path."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(5)
+    a = (torch.rand(m, k, device="cuda", generator=g) < 0.001).to(torch.bfloat16)
+    b = torch.randn(k, n, device="cuda", generator=g).to(torch.bfloat16)
+    for _ in range(3):
+        c = torch.matmul(a, b)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        c = torch.matmul(a, b)
+    e1.record()
+    torch.cuda.synchronize()
+    del a, b, c
+    return 2.0 * m * n * k * reps / (e0.elapsed_time(e1) * 1e-3) / 1e12
+
+
 def frozen_recall(batch: int = 8192, steps: int = 50, clamp_frac: float = 0.4) -> dict:
     """Train the 100x100 network (its training phase through the sweep of the
     first recall step), freeze W and bj, run `batch` cues for `steps` steps
@@ -558,6 +579,7 @@ def frozen_recall(batch: int = 8192, steps: int = 50, clamp_frac: float = 0.4) -
     for v in out.values():
         v.pop("idx")
     mf = out["mfma"]
+    vendor = vendor_gemm_tflops(batch, spec.total_units, spec.total_units)
     return {"workload": "100x100 network trained %d steps, W and bj frozen (bf16 for mfma), "
                         "%d cues (%.0f%% of HCUs clamped) x %d lock-step steps"
                         % (trained, batch, 100 * clamp_frac, steps),
@@ -566,7 +588,10 @@ def frozen_recall(batch: int = 8192, steps: int = 50, clamp_frac: float = 0.4) -
             "mfma_roofline": {"bound": "mfma", "achieved": mf.get("mfma_tflops"),
                               "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": (mf["mfma_tflops"] / MFMA_BF16_PEAK_TFLOPS
-                                       if mf.get("mfma_tflops") else None)}}
+                                       if mf.get("mfma_tflops") else None),
+                              "vendor_library_tflops": vendor,
+                              "vs_vendor_library": (mf["mfma_tflops"] / vendor
+                                                    if mf.get("mfma_tflops") else None)}}
 
 
 def main():
