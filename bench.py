@@ -629,6 +629,10 @@ def main():
               "hypercolumn is a design, not built). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    # torch before the library: it brings its own HIP runtime, and
+    # libgsbn_amd.so then binds to that one (same soname); the other way round
+    # the process would hold two runtimes and torch would find no device
+    import torch  # noqa: F401
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     spec, desc = workload(args.workload)
