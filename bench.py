@@ -104,6 +104,12 @@ def apply_plan(spec, plan: str):
     elif plan == "wave":
         spec.rowcol_wave = 1
         spec.chained_step = -1
+    elif plan.startswith("graded-wave:"):
+        spec.rowcol_wave = 1 + int(plan.split(":", 1)[1])
+        spec.chained_step = -1
+    elif plan.startswith("graded:"):
+        spec.rowcol_graded = int(plan.split(":", 1)[1])
+        spec.chained_step = -1
     elif plan.startswith("split:"):
         spec.rowcol_parts = int(plan.split(":", 1)[1])
         spec.chained_step = -1
@@ -204,6 +210,8 @@ PLAN_NAMES = {-1: "fused", -2: "chained", -20: "wave"}
 def plan_name(parts: int) -> str:
     if -20 < parts <= -10:
         return "graded-%d" % (-10 - parts)
+    if -24 < parts <= -21:
+        return "graded-wave-%d" % (-20 - parts)
     return PLAN_NAMES.get(parts, "split:%d" % parts)
 
 
@@ -218,8 +226,8 @@ def fix_plan(spec, parts: int):
         spec.fused_step = 1
     elif parts == -2:
         spec.chained_step = 1
-    elif parts == -20:
-        spec.rowcol_wave = 1
+    elif parts <= -20:
+        spec.rowcol_wave = -19 - parts
         spec.chained_step = -1
     elif parts <= -10:
         spec.rowcol_graded = -10 - parts
@@ -611,7 +619,9 @@ def main():
                          "(reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--plan", default="auto",
                     help="step plan: auto (tuned online), split:N (N row/column workgroups "
-                         "per HCU), fused, chained, or wave (batches: one wave per HCU)")
+                         "per HCU), fused, chained, or for batches wave (one wave per HCU), "
+                         "graded:G (parts per size bucket, G = 1-3) or graded-wave:G (the "
+                         "same with one wave per part)")
     ap.add_argument("--deferred", type=int, default=None, metavar="K",
                     help="deferred column updates with rings of K spike steps per unit "
                          "(ProcUpdLazy deferred-columns; 0: eager; default: the library's, 8 for one network, 16 for a batch)")

@@ -143,7 +143,8 @@ class NetSpec:
     # bucket of hypercolumns (plan -10-g); 0: chosen online
     rowcol_graded: int = 0
     # rowcol_wave 1: batches run their row/column update with one wave per
-    # hypercolumn (plan -20); 0: chosen online
+    # hypercolumn (plan -20); 2-4: one wave per part of the graded split
+    # g = rowcol_wave - 1 (plans -21 to -23); 0: chosen online
     rowcol_wave: int = 0
     # deferred_columns K (a power of two, 2..64): column updates deferred
     # through rings of K spike steps per postsynaptic unit; 0: eager; None:

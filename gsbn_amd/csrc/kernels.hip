@@ -1497,6 +1497,25 @@ __global__ void __launch_bounds__(64, RC_WAVES(PREC))
   }
 }
 
+/* The one-wave form with a work split per hypercolumn (the graded maps
+ * below): the hypercolumns with the most slots get several waves, each
+ * taking every P-th active row and a 1/P slice of the column walks, so that
+ * the few with many active rows in a step no longer end the launch alone. */
+template <int PREC>
+__global__ void __launch_bounds__(64, RC_WAVES(PREC))
+    rowcol_kernel_graded_wave(const gsbn_rowcol_args* __restrict__ arr,
+                              const int32_t* __restrict__ wg_proj,
+                              const int32_t* __restrict__ wg_hcu,
+                              const int32_t* __restrict__ wg_part, int nwg, gsbn_step_args st) {
+  for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
+    const gsbn_rowcol_args& a = arr[wg_proj[w]];
+    const int pp = wg_part[w];
+    const RcStep q = {st.simstep, st.exec, pp >> 16, a.table_lens[st.kp_slot], a.taupdt * st.prn,
+                      a.tables[st.kp_slot]};
+    rowcol_body<PREC, false, true, 64>(a, q, wg_hcu[w], pp & 0xffff);
+  }
+}
+
 /* The same with a work split per hypercolumn: workgroup w runs part
  * wg_part[w] & 0xffff of wg_part[w] >> 16 of hypercolumn wg_hcu[w] of
  * projection wg_proj[w] (larger hypercolumns split into more parts). */
@@ -2193,6 +2212,34 @@ int gsbn_proj_rowcol_graded(const gsbn_rowcol_args* projs_dev, const int32_t* wg
       launch_timed(rowcol_kernel_graded<GSBN_PREC_F16>,
                    dim3(loop_grid(rowcol_kernel_graded<GSBN_PREC_F16>, RC_THREADS, 0, nwg)), b,
                    0, s, projs_dev, wg_proj, wg_hcu, wg_part, nwg, *step);
+      break;
+  }
+  return launch_status();
+}
+
+int gsbn_proj_rowcol_graded_wave(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
+                                  const int32_t* wg_hcu, const int32_t* wg_part, int32_t nwg,
+                                  const gsbn_step_args* step, void* stream) {
+  if (!projs_dev || !wg_proj || !wg_hcu || !wg_part || !step || nwg <= 0 || step->kp_slot < 0 ||
+      step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 || step->prec > GSBN_PREC_F16)
+    return GSBN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 b(64);
+  switch (step->prec) {
+    case GSBN_PREC_F32:
+      launch_timed(rowcol_kernel_graded_wave<GSBN_PREC_F32>,
+                   dim3(loop_grid(rowcol_kernel_graded_wave<GSBN_PREC_F32>, 64, 0, nwg)), b, 0, s,
+                   projs_dev, wg_proj, wg_hcu, wg_part, nwg, *step);
+      break;
+    case GSBN_PREC_BF16:
+      launch_timed(rowcol_kernel_graded_wave<GSBN_PREC_BF16>,
+                   dim3(loop_grid(rowcol_kernel_graded_wave<GSBN_PREC_BF16>, 64, 0, nwg)), b, 0,
+                   s, projs_dev, wg_proj, wg_hcu, wg_part, nwg, *step);
+      break;
+    default:
+      launch_timed(rowcol_kernel_graded_wave<GSBN_PREC_F16>,
+                   dim3(loop_grid(rowcol_kernel_graded_wave<GSBN_PREC_F16>, 64, 0, nwg)), b, 0,
+                   s, projs_dev, wg_proj, wg_hcu, wg_part, nwg, *step);
       break;
   }
   return launch_status();

@@ -1274,7 +1274,8 @@ class ProcUpdLazy : public ProcedureBase {
   static constexpr int kTuneStart = 20, kTuneRun = 1, kTuneSamples = 10;
   int _parts_fixed = 0, _fused_fixed = 0;
   /* rowcol-wave 1: a batch's row/column update with one wave per
-   * hypercolumn (plan -20) */
+   * hypercolumn (plan -20); 2, 3, 4: one wave per part of the graded split
+   * g = 1, 2, 3 (plans -21, -22, -23) */
   int _wave_fixed = 0;
   /* deferred-columns K: column updates deferred through rings of K spike
    * steps per unit (0: eager); see Proj::dK.  Same results for every K, a
@@ -1341,7 +1342,7 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
   if (_fused_fixed) return -1;
   if (_chain_fixed > 0) return -2;
   if (_graded_fixed > 0) return -10 - _graded_fixed;
-  if (_wave_fixed > 0) return -20;
+  if (_wave_fixed > 0) return -19 - _wave_fixed;
   if (_parts_fixed > 0) return _parts_fixed;
   if (_tuned || execs < kTuneStart) return _parts;
   if (_cand.empty()) {
@@ -1360,7 +1361,10 @@ int ProcUpdLazy::rowcol_parts(hipStream_t s, bool* timed) {
     if (_fusable) _cand.push_back(-1);
     if (_chainable) _cand.push_back(-2);
     if (_batch && !_hcu_work.empty()) _cand.push_back(-20);
-    /* graded splits; their maps are built here, before any sample is timed */
+    /* graded splits; their maps are built here, before any sample is timed
+     * (their one-wave forms, plans -21 to -23, are no candidates: on 512
+     * networks the best of them, -21, ran within 2 % of plan -20, and every
+     * candidate more spreads the tuner's samples thinner) */
     if (_batch && !_hcu_work.empty())
       for (int g = 1; g <= 3; ++g) {
         _cand.push_back(-10 - g);
@@ -1426,6 +1430,8 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   /* rowcol-graded g (1-3): always the graded split g of a batch */
   par.argi("rowcol-graded", _graded_fixed);
   par.argi("rowcol-wave", _wave_fixed);
+  if (_wave_fixed < 0 || _wave_fixed > 4)
+    throw Error(-2, "ProcUpdLazy: rowcol-wave must be in [0, 4]");
   par.argi("deferred-columns", _defer_arg);
   if (_defer_arg != -1 &&
       (_defer_arg < 0 || _defer_arg == 1 || _defer_arg > 64 || (_defer_arg & (_defer_arg - 1))))
@@ -1695,7 +1701,7 @@ void ProcUpdLazy::update_gpu() {
     _glv.puti("rowcol-parts", _fused_fixed ? -1
                               : _chain_fixed > 0 ? -2
                               : _graded_fixed > 0 ? -10 - _graded_fixed
-                              : _wave_fixed > 0 ? -20
+                              : _wave_fixed > 0 ? -19 - _wave_fixed
                               : _parts_fixed > 0 ? _parts_fixed : _parts);
     /* GlobalVar plan-tuned: 1 once the plan no longer changes */
     _glv.puti("plan-tuned", (_fused_fixed || _chain_fixed > 0 || _graded_fixed > 0 ||
@@ -1853,7 +1859,13 @@ void ProcUpdLazy::update_gpu() {
     }
     if (!_list_proj.empty()) {
       Timing::get().begin(1, stream);
-      if (_batch && plan == -20) {
+      if (_batch && plan <= -21 && plan >= -23) {
+        const int g = -20 - plan;
+        if (!_gmap_n[g]) build_graded_map(g);
+        op_check(gsbn_proj_rowcol_graded_wave(_proj_dev.p, _gmap_proj[g].p, _gmap_hcu[g].p,
+                                              _gmap_part[g].p, _gmap_n[g], &st, stream),
+                 "gsbn_proj_rowcol_graded_wave");
+      } else if (_batch && plan == -20) {
         op_check(gsbn_proj_rowcol_batch_wave(_proj_dev.p, _wg_proj.p, _wg_proj_hcu.p, _nhcu_proj,
                                              &st, stream),
                  "gsbn_proj_rowcol_batch_wave");

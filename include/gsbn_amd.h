@@ -323,6 +323,14 @@ int gsbn_proj_rowcol_graded(const gsbn_rowcol_args* projs_dev, const int32_t* wg
                             const int32_t* wg_hcu, const int32_t* wg_part, int32_t nwg,
                             const gsbn_step_args* step, void* stream);
 
+/* gsbn_proj_rowcol_graded with one 64-thread workgroup (one wave) per map
+ * entry (Proj.cu:488-583 for every projection of a batch): the one-wave
+ * form with more waves for the hypercolumns with the most slots, whose many
+ * active rows in a step would otherwise end the launch alone. */
+int gsbn_proj_rowcol_graded_wave(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
+                                 const int32_t* wg_hcu, const int32_t* wg_part, int32_t nwg,
+                                 const gsbn_step_args* step, void* stream);
+
 /* One launch per executed step without a sweep (ProcUpdLazy.cpp:112-123
  * and 153-165 in one kernel), for populations with one incoming projection
  * and M <= 256: workgroup h runs the population update of hypercolumn h and

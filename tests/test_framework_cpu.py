@@ -115,6 +115,7 @@ def test_solver_without_device_fails_loudly(tmp_path):
 @pytest.mark.parametrize("field,key,value", [("chained_step", "chained-step", 1),
                                              ("chained_step", "chained-step", -1),
                                              ("rowcol_graded", "rowcol-graded", 2),
+                                             ("rowcol_wave", "rowcol-wave", 3),
                                              ("fused_step", "fused-step", 1),
                                              ("rowcol_parts", "rowcol-parts", 15),
                                              ("resident_stimulus", "resident-stimulus", 0)])

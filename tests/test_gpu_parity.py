@@ -263,13 +263,16 @@ def test_batch512_sampled_networks_bitwise(tmp_path):
     ora.close()
 
 
-def test_batched_networks_wave_plan(tmp_path):
-    # the batch's row/column update with one wave per hypercolumn (plan -20)
+@pytest.mark.parametrize("wave", [1, 2, 4])
+def test_batched_networks_wave_plan(tmp_path, wave):
+    # the batch's row/column update with one wave per hypercolumn (plan -20),
+    # or one wave per part of the graded splits g = 1 and 3 (plans -21, -23:
+    # the largest quarter of the hypercolumns in 2 resp. 8 waves)
     spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=(3, 140), n_pat=4, train=40,
                                 quiet=5, recall=30, reset=20)
-    spec.rowcol_wave = 1
+    spec.rowcol_wave = wave
     gpu, ora = _lockstep_and_compare(spec, tmp_path)
     c, t, flags = gpu.check()
     assert (c, t) == (ora.check.correct, ora.check.total)
-    assert gpu.glob("rowcol-parts") == -20
+    assert gpu.glob("rowcol-parts") == -19 - wave
     gpu.close()
