@@ -89,12 +89,14 @@ def workload(name: str):
 
 
 DEFERRED = None  # --deferred K, applied with the plan to every measured spec
+POP_WAVE = None  # --pop-block: 0 (the block population kernel for batches)
 
 
 def apply_plan(spec, plan: str):
     """--plan: auto leaves the choice to the online tuner; --deferred K
     defers the column updates."""
     spec.deferred_columns = DEFERRED
+    spec.pop_wave = POP_WAVE
     if plan == "auto":
         return
     if plan == "fused":
@@ -302,6 +304,7 @@ def measure(spec, steps, warmup, step_times=0):
         s.sync()
         parts = s.glob("rowcol-parts")
         deferred = s.glob("deferred-columns")
+        pop_wave = s.glob("pop-wave")
         t0 = time.perf_counter()
         done = s.step(steps)
         s.sync()
@@ -344,6 +347,7 @@ def measure(spec, steps, warmup, step_times=0):
     batch = len(spec.pops) > 1
     rc_name = (("chained_kernel" if parts == -2 else "step_kernel_batch" if batch else "step_kernel")
                if one_launch else "rowcol_kernel_batch_wave" if parts == -20
+               else "rowcol_kernel_graded_wave" if parts <= -21
                else "rowcol_kernel_graded" if parts <= -10
                else "rowcol_kernel_batch" if batch else "rowcol_kernel")
     kind = 6 if one_launch else 1
@@ -361,7 +365,8 @@ def measure(spec, steps, warmup, step_times=0):
         pop_bytes += POP_BYTES_PER_CONTRIB * rows / int(launches[0])
     kernels[rc_name] = (n_rc, float(ms[kind]), rc_bytes)
     if not one_launch and int(launches[0]):
-        kernels["pop_kernel_batch" if batch else "pop_kernel"] = (
+        kernels[("pop_kernel_batch_wave" if pop_wave > 0 else "pop_kernel_batch") if batch
+                else "pop_kernel"] = (
             int(launches[0]), float(ms[0]), float(pop_bytes))
     total_ms = sum(v[1] for v in kernels.values())
     table = {}
@@ -503,6 +508,7 @@ def batched(steps: int, warmup: int, form: str = "fp32") -> dict:
     """The 512-network form: aggregate and per-network step rates."""
     spec, desc = workload("batch512")
     spec.deferred_columns = DEFERRED
+    spec.pop_wave = POP_WAVE
     spec.trace_precision = form
     if form != "fp32":
         desc = desc.replace("plain FP32 state", "Pij/Eij stored in " + form)
@@ -625,13 +631,17 @@ def main():
     ap.add_argument("--deferred", type=int, default=None, metavar="K",
                     help="deferred column updates with rings of K spike steps per unit "
                          "(ProcUpdLazy deferred-columns; 0: eager; default: the library's, 8 for one network, 16 for a batch)")
+    ap.add_argument("--pop-block", action="store_true",
+                    help="batches: the population update with one workgroup per hypercolumn "
+                         "(ProcUpdLazy pop-wave 0) instead of one wave")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--cpu-arith", default="ref32", choices=("ref32", "table"))
     ap.add_argument("--step-times", type=int, default=0,
                     help="after the timed region, time this many synchronised single steps")
     args = ap.parse_args()
-    global DEFERRED
+    global DEFERRED, POP_WAVE
     DEFERRED = args.deferred
+    POP_WAVE = 0 if args.pop_block else None
     if args.cpu_only:
         return cpu_only(args)
     if args.gpus != 1:

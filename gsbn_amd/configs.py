@@ -146,6 +146,10 @@ class NetSpec:
     # hypercolumn (plan -20); 2-4: one wave per part of the graded split
     # g = rowcol_wave - 1 (plans -21 to -23); 0: chosen online
     rowcol_wave: int = 0
+    # pop_wave 0: batches run their population update with the block kernel;
+    # None: ProcUpdLazy's default (one wave per hypercolumn where every
+    # population qualifies)
+    pop_wave: Optional[int] = None
     # deferred_columns K (a power of two, 2..64): column updates deferred
     # through rings of K spike steps per postsynaptic unit; 0: eager; None:
     # ProcUpdLazy's default (where it applies: 8 for one network, 16 for a batch)
@@ -319,6 +323,8 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('  argi { key: "rowcol-wave" val: %d }' % spec.rowcol_wave)
     if spec.deferred_columns is not None:
         L.append('  argi { key: "deferred-columns" val: %d }' % spec.deferred_columns)
+    if spec.pop_wave is not None:
+        L.append('  argi { key: "pop-wave" val: %d }' % spec.pop_wave)
     L.append("}")
     if spec.conn_arrays is None:
         L.append("proc_param {")

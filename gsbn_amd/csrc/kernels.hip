@@ -1167,6 +1167,317 @@ __global__ void __launch_bounds__(POP_MAX_THREADS, 6)
   }
 }
 
+/* One hypercolumn's population update by one wave, UPL units per lane
+ * (unit j = lane + 64 u): pop_body's arithmetic, in the same order per unit,
+ * for the batched populations it covers — at most one incoming projection
+ * (its ring slot in at most four words per lane, C <= 8192) and one
+ * outgoing, no external spikes, no population-wide WTA maximum.  A
+ * 128-thread workgroup of pop_body spends two waves, kept together by every
+ * barrier, on an HCU of up to 128 units and leaves one idle for M <= 64;
+ * here each hypercolumn is one wave, and its lists are placed by wave scans
+ * taken unit block by unit block (units u = 0 first), which keeps them
+ * ascending.  LDS: three arrays of 64 UPL words. */
+template <int UPL>
+__device__ __forceinline__ void pop_body_w(const gsbn_pop_args& a, const PopStep& ps, const int h) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t sh_dyn[];
+  float* const sh_f = reinterpret_cast<float*>(sh_dyn);
+  int* const sh_ext = reinterpret_cast<int*>(sh_dyn + 64 * UPL);
+  int* const sh_spk = reinterpret_cast<int*>(sh_dyn + 128 * UPL);
+  const int lane = threadIdx.x;
+  const int M = a.M;
+  const int nin = a.nproj_in;
+  const gsbn_pop_proj& p0 = a.proj0;
+
+  /* 0. loads that do not depend on this step's results */
+  float dsup_old[UPL], lg[UPL], e0[UPL], bj0[UPL], pj0[UPL], ej0[UPL], zj0[UPL], cpre[UPL];
+  int32_t counter_old[UPL];
+  const float wmv = ps.wmask[h];
+  int pend0 = 0;
+  uint32_t fut[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    const int j = lane + 64 * u;
+    const int idx = h * M + j;
+    dsup_old[u] = lg[u] = e0[u] = bj0[u] = pj0[u] = ej0[u] = zj0[u] = cpre[u] = 0.0f;
+    counter_old[u] = 0;
+    if (j < M) {
+      dsup_old[u] = a.dsup[idx];
+      lg[u] = ps.lginp[idx];
+      counter_old[u] = a.counter[idx];
+      if (nin > 0) {
+        e0[u] = a.epsc[idx];
+        bj0[u] = a.bj[idx];
+        pj0[u] = p0.pj[idx];
+        ej0[u] = p0.ej[idx];
+        zj0[u] = p0.zj[idx];
+        if (p0.C > 0) cpre[u] = p0.contrib[((int64_t)h * p0.C) * M + j];
+      }
+    }
+  }
+  if (nin > 0) {
+    pend0 = p0.pend_cnt[h];
+    const uint32_t* wslot = p0.fut + ((int64_t)(ps.exec % GSBN_FUT) * a.H + h) * p0.W;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (lane + 64 * i < p0.W) fut[i] = wslot[lane + 64 * i];
+  }
+  /* the draws and the first projection's postsynaptic traces do not depend
+   * on this step's results either */
+  float ud[UPL], xi[UPL], pj_n[UPL], ej_n[UPL], zj_n[UPL], bj_n[UPL];
+  bool bj_w[UPL];
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    const int j = lane + 64 * u;
+    ud[u] = xi[u] = pj_n[u] = ej_n[u] = zj_n[u] = bj_n[u] = 0.0f;
+    bj_w[u] = false;
+    if (j < M) {
+      float z;
+      gsbn_draw_unit(a.seed, (uint32_t)a.pop_id, (uint32_t)ps.simstep, (uint32_t)(h * M + j), &ud[u],
+                     &z);
+      xi[u] = a.snoise * z;
+      if (ps.do_jxx && nin > 0) {
+        const float kp = p0.taupdt * ps.prn;
+        pj_n[u] = pj0[u] + (ej0[u] - pj0[u]) * kp;
+        ej_n[u] = ej0[u] + (zj0[u] - ej0[u]) * p0.ke;
+        zj_n[u] = zj0[u] * (1 - p0.kzj);
+        bj_w[u] = kp != 0.0f;
+        if (bj_w[u]) bj_n[u] = p0.bgain * gsbn_logf(pj_n[u] + p0.eps);
+      }
+    }
+  }
+
+  /* 1. pending support contributions, ascending row order; 2. dsup */
+  float efin[UPL], dsup[UPL];
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    const int j = lane + 64 * u;
+    efin[u] = 0.0f;
+    dsup[u] = 0.0f;
+    if (j >= M) continue;
+    if (nin > 0) {
+      float e = e0[u];
+      if (0 < pend0 && gsbn_fbits(cpre[u]) != GSBN_NO_CONTRIB_BITS) e = e + cpre[u];
+      const float* c = p0.contrib + ((int64_t)h * p0.C) * M + j;
+      for (int k = 1; k < pend0; k += 8) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = k + q < pend0 ? c[(int64_t)(k + q) * M] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (k + q < pend0 && gsbn_fbits(v[q]) != GSBN_NO_CONTRIB_BITS) e = e + v[q];
+      }
+      efin[u] = e;
+    }
+    float wsup = 0.0f;
+    if (nin > 0) wsup += bj0[u] + efin[u];
+    float sup = a.lgbias + a.igain * lg[u] + xi[u];
+    sup += (a.wgain * wmv) * wsup;
+    dsup[u] = dsup_old[u] + (sup - dsup_old[u]) * a.taumdt;
+  }
+
+  /* 3. WTA maximum (Pop.cpp:270-279), unit blocks in order */
+  float maxdsup = -INFINITY;
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    const float o = wave_max_left(lane + 64 * u < M ? dsup[u] : -INFINITY);
+    if (u == 0 || o > maxdsup) maxdsup = o;
+  }
+
+  /* soft-max (Pop.cpp:280-302), the normaliser in ascending unit order */
+  const float maxact = gsbn_expf(a.wtagain * maxdsup);
+  float act[UPL];
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    const int j = lane + 64 * u;
+    act[u] = 0.0f;
+    if (j < M) {
+      act[u] = gsbn_expf(a.wtagain * (dsup[u] - maxdsup));
+      if (maxact < 1.0f) act[u] *= maxact;
+      sh_f[j] = act[u];
+    }
+  }
+  __syncthreads();
+  float vsum = 0.0f;
+  if (lane == 0) vsum = ordered_sum_lds(sh_f, M);
+  vsum = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vsum)));
+
+  /* 4. spikes and counters (Pop.cpp:305-323); the fan-out range and ring
+   * state of the spiking units */
+  int spk[UPL], fo0[UPL], fo1[UPL], dcnt0[UPL], dfl0[UPL];
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    const int j = lane + 64 * u;
+    const int idx = h * M + j;
+    if (j < M && vsum > 1.0f) act[u] /= vsum;
+    spk[u] = (j < M && ud[u] < act[u] * a.maxfqdt) ? 1 : 0;
+    fo0[u] = fo1[u] = dcnt0[u] = dfl0[u] = 0;
+    if (spk[u]) {
+      if (a.nproj_out > 0) {
+        fo0[u] = a.out0.fwd_off[idx];
+        fo1[u] = a.out0.fwd_off[idx + 1];
+      }
+      if (nin > 0 && p0.dK) {
+        dcnt0[u] = p0.dcnt[idx];
+        dfl0[u] = p0.dfl[idx];
+      }
+    }
+  }
+
+  /* 5. positions: spikes | walks per unit block, then the set bits of the
+   * ring words (word lane + 64 i in band i) */
+  int pos_s[UPL], pos_w[UPL], wflag[UPL];
+  int spk_tot = 0, walk_tot = 0;
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    wflag[u] = (nin > 0 && p0.dK && spk[u] && dcnt0[u] + 1 - dfl0[u] >= p0.dK - 1) ? 1 : 0;
+    const uint32_t v = (uint32_t)spk[u] | ((uint32_t)wflag[u] << 16);
+    const uint32_t incl = wave_incl_scan(v);
+    const uint32_t ex = incl - v;
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    pos_s[u] = spk_tot + (int)(ex & 0xffffu);
+    pos_w[u] = walk_tot + (int)(ex >> 16);
+    spk_tot += (int)(tot & 0xffffu);
+    walk_tot += (int)(tot >> 16);
+  }
+  uint32_t band_pre[4], band_tot[4];
+  {
+    const uint32_t sb = (uint32_t)__popc(fut[0]) | ((uint32_t)__popc(fut[1]) << 16);
+    const uint32_t sc = (uint32_t)__popc(fut[2]) | ((uint32_t)__popc(fut[3]) << 16);
+    const uint32_t ib = wave_incl_scan(sb), ic = wave_incl_scan(sc);
+    const uint32_t eb = ib - sb, ec = ic - sc;
+    const uint32_t tb = (uint32_t)__builtin_amdgcn_readlane((int)ib, 63);
+    const uint32_t tc = (uint32_t)__builtin_amdgcn_readlane((int)ic, 63);
+    band_pre[0] = eb & 0xffffu;
+    band_pre[1] = eb >> 16;
+    band_pre[2] = ec & 0xffffu;
+    band_pre[3] = ec >> 16;
+    band_tot[0] = tb & 0xffffu;
+    band_tot[1] = tb >> 16;
+    band_tot[2] = tc & 0xffffu;
+    band_tot[3] = tc >> 16;
+  }
+  __syncthreads(); /* the normaliser's reads of sh_f are done */
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    if (!spk[u]) continue;
+    sh_spk[pos_s[u]] = lane + 64 * u;
+    sh_ext[pos_s[u]] = fo0[u];
+    sh_f[pos_s[u]] = __int_as_float(fo1[u]);
+  }
+
+  /* 5b. queue push of this step's spikes into the outgoing ring */
+  if (a.nproj_out > 0 && spk_tot > 0) {
+    __syncthreads();
+    for (int i = 0; i < spk_tot; ++i)
+      push_rows(a.out0, sh_ext[i], __float_as_int(sh_f[i]), ps.exec, lane, 64);
+  }
+
+  /* --- stores: the population's state (steps 1-4) --- */
+#pragma unroll
+  for (int u = 0; u < UPL; ++u) {
+    const int j = lane + 64 * u;
+    if (j >= M) continue;
+    const int idx = h * M + j;
+    a.dsup[idx] = dsup[u];
+    a.rnd_u[idx] = ud[u];
+    a.rnd_n[idx] = xi[u];
+    a.act[idx] = act[u];
+    a.counter[idx] = counter_old[u] + spk[u];
+    a.spike[idx] = (int8_t)spk[u];
+    if (spk[u]) a.spk_list[h * M + pos_s[u]] = j;
+  }
+  if (lane == 0) a.spk_cnt[h] = spk_tot;
+
+  if (nin > 0) {
+    /* 6. postsynaptic traces and bias (Proj.cpp:310-363) */
+#pragma unroll
+    for (int u = 0; u < UPL; ++u) {
+      const int j = lane + 64 * u;
+      if (j >= M) continue;
+      const int idx = h * M + j;
+      if (!ps.do_jxx) {
+        a.epsc[idx] = efin[u];
+      } else {
+        a.epsc[idx] = efin[u] * (1 - p0.kepsc);
+        float zj = zj_n[u];
+        if (spk[u] > 0) zj += p0.kftj;
+        if (bj_w[u]) a.bj[idx] = bj_n[u];
+        p0.pj[idx] = pj_n[u];
+        p0.ej[idx] = ej_n[u];
+        p0.zj[idx] = zj;
+      }
+    }
+    /* 6b. deferred column updates: ring pushes and this step's walks */
+    if (p0.dK) {
+      const int K = p0.dK;
+#pragma unroll
+      for (int u = 0; u < UPL; ++u) {
+        if (!spk[u]) continue;
+        const int j = lane + 64 * u;
+        const int idx = h * M + j;
+        const int n = dcnt0[u];
+        p0.dhist[(int64_t)idx * K + (n & (K - 1))] = ps.simstep;
+        p0.dcnt[idx] = n + 1;
+        if (wflag[u]) {
+          p0.dfl[idx] = n + 1;
+          p0.walk_list[h * M + pos_w[u]] = j;
+          p0.walk_T[h * M + pos_w[u]] = n + 1;
+        }
+      }
+      if (lane == 0) p0.walk_cnt[h] = walk_tot;
+    }
+    /* 7. active rows of this step (Proj.cpp:621-630) from the ring words */
+    const int C = p0.C;
+    uint32_t* words = p0.fut + ((int64_t)(ps.exec % GSBN_FUT) * a.H + h) * p0.W;
+    uint32_t base = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int w = lane + 64 * i;
+      uint32_t bits = fut[i];
+      if (bits) {
+        words[w] = 0u;
+        int pos = (int)(base + band_pre[i]);
+        while (bits) {
+          const int b = __ffs(bits) - 1;
+          bits &= bits - 1;
+          const int k = (w << 5) + b;
+          p0.rows_list[(int64_t)h * C + pos] = k;
+          p0.act_stamp[(int64_t)h * C + k] = ps.exec;
+          ++pos;
+        }
+      }
+      base += band_tot[i];
+    }
+    if (lane == 0) {
+      const int tot = (int)base;
+      p0.rows_cnt[h] = tot;
+      p0.pend_cnt[h] = tot;
+      if (a.work) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.work[0]),
+                  (unsigned long long)tot * (unsigned long long)M);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.work[1]),
+                  (unsigned long long)spk_tot * (unsigned long long)C);
+      }
+    }
+  }
+}
+
+/* pop_kernel_batch with one 64-thread workgroup (one wave) per hypercolumn
+ * of the map, through pop_body_w; UPL = 2 covers M <= 128.  Six waves per
+ * SIMD (80 registers, no spills): 24 hypercolumns in flight per CU against
+ * the block kernel's 12. */
+template <int UPL>
+__global__ void __launch_bounds__(64, 6)
+    pop_kernel_batch_wave(const gsbn_pop_args* __restrict__ arr, const int32_t* __restrict__ wg_pop,
+                          const int32_t* __restrict__ wg_hcu, int nwg, gsbn_step_args st) {
+  for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads();
+    const gsbn_pop_args& a = arr[wg_pop[w]];
+    touch_lines<sizeof(gsbn_pop_args)>(&a);
+    pop_body_w<UPL>(a, pop_step(a, st), wg_hcu[w]);
+  }
+}
+
 __global__ void pop_dsup0_kernel_batch(const gsbn_pop_args* __restrict__ arr, int npop,
                                        gsbn_step_args st) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2124,6 +2435,23 @@ int gsbn_pop_update_batch(const gsbn_pop_args* pops_dev, int32_t npop, const int
   }
   launch_timed(pop_kernel_batch, dim3(loop_grid(pop_kernel_batch, threads, lds, nwg)),
                dim3(threads), lds, s, pops_dev, wg_pop, wg_hcu, nwg, *step);
+  return launch_status();
+}
+
+int gsbn_pop_update_batch_wave(const gsbn_pop_args* pops_dev, const int32_t* wg_pop,
+                               const int32_t* wg_hcu, int32_t nwg, int32_t units_per_lane,
+                               const gsbn_step_args* step, void* stream) {
+  if (!pops_dev || !wg_pop || !wg_hcu || !step || nwg <= 0 ||
+      (units_per_lane != 1 && units_per_lane != 2))
+    return GSBN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t lds = (size_t)3 * 64 * units_per_lane * sizeof(uint32_t);
+  if (units_per_lane == 1)
+    launch_timed(pop_kernel_batch_wave<1>, dim3(loop_grid(pop_kernel_batch_wave<1>, 64, lds, nwg)),
+                 dim3(64), lds, s, pops_dev, wg_pop, wg_hcu, nwg, *step);
+  else
+    launch_timed(pop_kernel_batch_wave<2>, dim3(loop_grid(pop_kernel_batch_wave<2>, 64, lds, nwg)),
+                 dim3(64), lds, s, pops_dev, wg_pop, wg_hcu, nwg, *step);
   return launch_status();
 }
 

@@ -1235,6 +1235,10 @@ class ProcUpdLazy : public ProcedureBase {
   vector<uint8_t> _pop_sent, _proj_sent; /* what the two arrays hold on the device */
   DevBuf<int32_t> _wg_pop, _wg_pop_hcu, _wg_proj, _wg_proj_hcu, _in_proj;
   int _pop_threads = 64, _max_slots = 0, _any_quirk = 0, _nwg_pop = 0, _nhcu_proj = 0, _parts = 1;
+  /* pop-wave 1 (default): a batch's population update with one wave per
+   * hypercolumn (gsbn_pop_update_batch_wave, _pop_upl units per lane) when
+   * every population qualifies; 0: the block kernel */
+  int _pop_wave = 1, _pop_upl = 0;
 
   int _max_m = 0, _parts0 = 1;
   /* Chained step (plan -2): one launch per step runs the previous step's
@@ -1430,6 +1434,7 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   /* rowcol-graded g (1-3): always the graded split g of a batch */
   par.argi("rowcol-graded", _graded_fixed);
   par.argi("rowcol-wave", _wave_fixed);
+  par.argi("pop-wave", _pop_wave);
   if (_wave_fixed < 0 || _wave_fixed > 4)
     throw Error(-2, "ProcUpdLazy: rowcol-wave must be in [0, 4]");
   par.argi("deferred-columns", _defer_arg);
@@ -1505,6 +1510,17 @@ void ProcUpdLazy::build_batch_maps() {
       wh.push_back(h);
     }
     _pop_threads = std::max(_pop_threads, ((p->_dim_mcu + 63) / 64) * 64);
+  }
+  _pop_upl = 0;
+  if (_pop_wave && !_wta_pop_max) {
+    bool ok = true;
+    int maxm = 0;
+    for (Pop* p : _list_pop) {
+      maxm = std::max(maxm, p->_dim_mcu);
+      if (p->_dim_mcu > 128 || p->_in.size() > 1 || p->_out.size() > 1 || p->ext_off.p) ok = false;
+      if (!p->_in.empty() && p->_in[0]->_dim_conn > 8192) ok = false;
+    }
+    if (ok) _pop_upl = maxm <= 64 ? 1 : 2;
   }
   /* row/column workgroups of the largest hypercolumns (slots x MCUs, the
    * size of a column pass and of a row) first: the long ones start early
@@ -1693,6 +1709,9 @@ void ProcUpdLazy::update_gpu() {
       }
     }
     _glv.puti("deferred-columns", _defer_k);
+    /* GlobalVar pop-wave: units per lane of the one-wave population launch
+     * of a batch (0: the block kernel, or no batch) */
+    _glv.puti("pop-wave", _batch ? _pop_upl : 0);
     if (_graded_fixed && !_batch)
       throw Error(-2, "ProcUpdLazy: rowcol-graded needs a batch of populations or projections");
     if (_wave_fixed > 0 && !_batch)
@@ -1840,7 +1859,11 @@ void ProcUpdLazy::update_gpu() {
   } else {
     for (auto* p : _list_pop) p->spk_side = 0;
     Timing::get().begin(0, stream);
-    if (_batch)
+    if (_batch && _pop_upl)
+      op_check(gsbn_pop_update_batch_wave(_pop_dev.p, _wg_pop.p, _wg_pop_hcu.p, _nwg_pop, _pop_upl,
+                                          &st, stream),
+               "gsbn_pop_update_batch_wave");
+    else if (_batch)
       op_check(gsbn_pop_update_batch(_pop_dev.p, (int32_t)_pop_args.size(), _wg_pop.p,
                                      _wg_pop_hcu.p, _nwg_pop, _pop_threads, _max_slots, _any_quirk,
                                      &st, stream),

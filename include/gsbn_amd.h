@@ -305,6 +305,16 @@ int gsbn_proj_rowcol_batch(const gsbn_rowcol_args* projs_dev, const int32_t* wg_
                            const int32_t* wg_hcu, int32_t nhcu, int32_t parts,
                            const gsbn_step_args* step, void* stream);
 
+/* gsbn_pop_update_batch (Pop.cu:8-108 and the queue push, jxx and active-row
+ * steps of gsbn_pop_update) with one wave per hypercolumn, units_per_lane
+ * (1 or 2) units per lane: same results, for batches whose every population
+ * has M <= 64 * units_per_lane, at most one incoming projection with C <=
+ * 8192 slots, at most one outgoing projection, no external spikes and no
+ * wta_pop_max (the caller checks; gsbn_pop_update_batch covers the rest). */
+int gsbn_pop_update_batch_wave(const gsbn_pop_args* pops_dev, const int32_t* wg_pop,
+                               const int32_t* wg_hcu, int32_t nwg, int32_t units_per_lane,
+                               const gsbn_step_args* step, void* stream);
+
 /* gsbn_proj_rowcol_batch with one 64-thread workgroup (one wave) per
  * hypercolumn of the map: same results, more hypercolumns in flight when
  * each has little work. */

@@ -134,11 +134,15 @@ def test_config1_tuned_plans_state_every_100_steps(tmp_path):
     gpu.close()
 
 
-def test_batched_networks_bitwise(tmp_path):
+@pytest.mark.parametrize("pop_wave,mcu", [(None, (3, 17)), (0, (3, 17)), (None, (3, 128))])
+def test_batched_networks_bitwise(tmp_path, pop_wave, mcu):
     # independent networks of different sizes and connection densities in one
-    # batched population launch and one batched row/column launch
-    spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=(3, 17), n_pat=4, train=40,
+    # batched population launch and one batched row/column launch; the
+    # population launch one wave per hypercolumn (one or two units per lane)
+    # or, pop_wave 0, one workgroup per hypercolumn
+    spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=mcu, n_pat=4, train=40,
                                 quiet=5, recall=30, reset=20)
+    spec.pop_wave = pop_wave
     gpu, ora = _lockstep_and_compare(spec, tmp_path)
     c, t, flags = gpu.check()
     assert (c, t) == (ora.check.correct, ora.check.total)
