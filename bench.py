@@ -90,6 +90,7 @@ def workload(name: str):
 
 DEFERRED = None  # --deferred K, applied with the plan to every measured spec
 POP_WAVE = None  # --pop-block: 0 (the block population kernel for batches)
+OVF_KEEP = None  # --ovf-keep K: the one-wave row/column plan's row overflow queue
 
 
 def apply_plan(spec, plan: str):
@@ -97,6 +98,7 @@ def apply_plan(spec, plan: str):
     defers the column updates."""
     spec.deferred_columns = DEFERRED
     spec.pop_wave = POP_WAVE
+    spec.ovf_keep = OVF_KEEP
     if plan == "auto":
         return
     if plan == "fused":
@@ -509,6 +511,7 @@ def batched(steps: int, warmup: int, form: str = "fp32") -> dict:
     spec, desc = workload("batch512")
     spec.deferred_columns = DEFERRED
     spec.pop_wave = POP_WAVE
+    spec.ovf_keep = OVF_KEEP
     spec.trace_precision = form
     if form != "fp32":
         desc = desc.replace("plain FP32 state", "Pij/Eij stored in " + form)
@@ -634,14 +637,19 @@ def main():
     ap.add_argument("--pop-block", action="store_true",
                     help="batches: the population update with one workgroup per hypercolumn "
                          "(ProcUpdLazy pop-wave 0) instead of one wave")
+    ap.add_argument("--ovf-keep", type=int, default=None, metavar="K",
+                    help="batches, one-wave row/column plan: active rows a hypercolumn keeps "
+                         "for its own wave, the rest queued for helper waves (ProcUpdLazy "
+                         "ovf-keep; 0: no queue; default: the library's, 2)")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--cpu-arith", default="ref32", choices=("ref32", "table"))
     ap.add_argument("--step-times", type=int, default=0,
                     help="after the timed region, time this many synchronised single steps")
     args = ap.parse_args()
-    global DEFERRED, POP_WAVE
+    global DEFERRED, POP_WAVE, OVF_KEEP
     DEFERRED = args.deferred
     POP_WAVE = 0 if args.pop_block else None
+    OVF_KEEP = args.ovf_keep
     if args.cpu_only:
         return cpu_only(args)
     if args.gpus != 1:

@@ -150,6 +150,11 @@ class NetSpec:
     # None: ProcUpdLazy's default (one wave per hypercolumn where every
     # population qualifies)
     pop_wave: Optional[int] = None
+    # ovf_keep / ovf_cap: the one-wave row/column plan's row overflow queue
+    # (rows a hypercolumn keeps for its own wave, queued rows per step at
+    # most; ovf_keep 0: no queue); None: ProcUpdLazy's defaults (2, 2^18)
+    ovf_keep: Optional[int] = None
+    ovf_cap: Optional[int] = None
     # deferred_columns K (a power of two, 2..64): column updates deferred
     # through rings of K spike steps per postsynaptic unit; 0: eager; None:
     # ProcUpdLazy's default (where it applies: 8 for one network, 16 for a batch)
@@ -325,6 +330,10 @@ def prototxt(spec: NetSpec, stim_path: str, conn_path: str, rec_dir: str,
         L.append('  argi { key: "deferred-columns" val: %d }' % spec.deferred_columns)
     if spec.pop_wave is not None:
         L.append('  argi { key: "pop-wave" val: %d }' % spec.pop_wave)
+    if spec.ovf_keep is not None:
+        L.append('  argi { key: "ovf-keep" val: %d }' % spec.ovf_keep)
+    if spec.ovf_cap is not None:
+        L.append('  argi { key: "ovf-cap" val: %d }' % spec.ovf_cap)
     L.append("}")
     if spec.conn_arrays is None:
         L.append("proc_param {")

@@ -45,7 +45,21 @@ static_assert(sizeof(gsbn_syn16_rec) == 16 && sizeof(gsbn_syn16_t) == 16,
 #define POP_MAX_THREADS 1024
 #define RC_THREADS 256
 #define RC_COLS 4 /* column slots per thread and pass */
-#define GSBN_MAX_DEFER 64 /* largest ring of deferred column kicks per unit */
+#define GSBN_MAX_DEFER 64
+#define GSBN_OVF_HELPERS 4096 /* helper waves of a row/column launch with a row overflow queue */
+#define GSBN_OVF_QUEUES 64    /* its sub-queues, one counter each (64-byte stride) */
+
+/* the overflow sub-queue of hypercolumn h of batch projection p, and the
+ * word of its counter for a step's parity */
+__device__ __forceinline__ int ovf_queue(int p, int h) {
+  uint32_t x = ((uint32_t)p * 0x9E3779B1u) ^ ((uint32_t)h * 0x85EBCA6Bu);
+  x ^= x >> 13;
+  x *= 0xC2B2AE35u;
+  return (int)(x >> 26);
+}
+__device__ __forceinline__ int ovf_counter(int exec, int q) {
+  return ((exec & 1) * GSBN_OVF_QUEUES + q) * 16;
+} /* largest ring of deferred column kicks per unit */
 #define ALL_THREADS 256
 #define ALL_ROWS 16
 
@@ -618,15 +632,18 @@ struct PopStep {
   float prn;
   const float* lginp;
   const float* wmask;
+  int ovf_keep, ovf_cap; /* the row overflow queue (gsbn_step_args) */
+  int32_t* ovf_cnt;
+  int32_t* ovf_task;
 };
 
 __device__ __forceinline__ PopStep pop_step(const gsbn_pop_args& a) {
-  return {a.simstep, a.exec, a.do_jxx, a.prn, a.lginp, a.wmask};
+  return {a.simstep, a.exec, a.do_jxx, a.prn, a.lginp, a.wmask, 0, 0, nullptr, nullptr};
 }
 
 __device__ __forceinline__ PopStep pop_step(const gsbn_pop_args& a, const gsbn_step_args& st) {
   return {st.simstep, st.exec, st.do_jxx, st.prn, a.lginp + st.lginp_offset,
-          a.wmask + st.wmask_offset};
+          a.wmask + st.wmask_offset, st.ovf_keep, st.ovf_cap, st.ovf_cnt, st.ovf_task};
 }
 
 /* One hypercolumn's population update.  FUSED: `f` also receives what the
@@ -1459,6 +1476,27 @@ __device__ __forceinline__ void pop_body_w(const gsbn_pop_args& a, const PopStep
                   (unsigned long long)spk_tot * (unsigned long long)C);
       }
     }
+    /* rows past the first ovf_keep go to this hypercolumn's overflow
+     * sub-queue, one task each (tasks past the sub-queue's share of ovf_cap
+     * are not written: the row/column launch then runs every row of that
+     * sub-queue's hypercolumns in their own waves) */
+    const int keep = ps.ovf_keep;
+    if (keep > 0 && (int)base > keep) {
+      const int n = (int)base - keep;
+      const int qq = ovf_queue(p0.bidx, h);
+      const int capq = ps.ovf_cap / GSBN_OVF_QUEUES;
+      int b0 = 0;
+      if (lane == 0) b0 = atomicAdd(&ps.ovf_cnt[ovf_counter(ps.exec, qq)], n);
+      b0 = __builtin_amdgcn_readfirstlane(b0);
+      for (int i = lane; i < n; i += 64) {
+        if (b0 + i < capq) {
+          const int64_t t = (int64_t)qq * capq + b0 + i;
+          ps.ovf_task[3 * t] = p0.bidx;
+          ps.ovf_task[3 * t + 1] = h;
+          ps.ovf_task[3 * t + 2] = keep + i;
+        }
+      }
+    }
   }
 }
 
@@ -1470,6 +1508,10 @@ template <int UPL>
 __global__ void __launch_bounds__(64, 6)
     pop_kernel_batch_wave(const gsbn_pop_args* __restrict__ arr, const int32_t* __restrict__ wg_pop,
                           const int32_t* __restrict__ wg_hcu, int nwg, gsbn_step_args st) {
+  /* the overflow counter of the next step (the row/column launch of the
+   * previous step, which read it, has finished) */
+  if (st.ovf_cnt && blockIdx.x == 0 && threadIdx.x < GSBN_OVF_QUEUES)
+    st.ovf_cnt[ovf_counter(st.exec + 1, threadIdx.x)] = 0;
   for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
     if (w != (int)blockIdx.x) __syncthreads();
     const gsbn_pop_args& a = arr[wg_pop[w]];
@@ -1511,7 +1553,8 @@ __global__ void __launch_bounds__(POP_MAX_THREADS) pop_flush_kernel(gsbn_pop_arg
 template <int PREC, bool FUSED = false, bool ROWS = true, int NT = RC_THREADS, bool RING8 = false>
 __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcStep& q,
                                             const int h, const int part,
-                                            const FusedLds* fl = nullptr) {
+                                            const FusedLds* fl = nullptr, const int keep = INT_MAX,
+                                            const int row_first = 0, const bool one_row = false) {
   const SynIO<PREC> io(a);
   const int P = q.P;
   const int M = a.M, C = a.C;
@@ -1523,17 +1566,20 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
    * counts, this thread's first row and the first spiking unit are read in
    * one batch, before either is known to be needed (all inside their
    * arrays). */
+  /* keep: the rows this call takes at most (the rest are queued tasks);
+   * one_row: only row position row_first, no column work (a queued task) */
   const int rpt = M >= NT ? 1 : NT / M;
   int nrows, nspk, k_pre = 0, j_pre = 0;
   if (FUSED) {
     nrows = fl->nrows;
     nspk = fl->nspk;
   } else {
-    nrows = ROWS ? a.rows_cnt[h] : 0;
-    nspk = a.spk_cnt[h];
-    const int aa0 = part + P * (tid / M);
+    nrows = one_row ? 1 : ROWS ? a.rows_cnt[h] : 0;
+    if (nrows > keep) nrows = keep;
+    nspk = one_row ? 0 : a.spk_cnt[h];
+    const int aa0 = row_first + part + P * (tid / M);
     if (ROWS && tid / M < rpt && aa0 < C) k_pre = a.rows_list[(int64_t)h * C + aa0];
-    j_pre = a.spk_list[h * M];
+    if (!one_row) j_pre = a.spk_list[h * M];
   }
   const int mine = nrows > part ? (nrows - part + P - 1) / P : 0;
   for (int base = 0; base < mine; base += rpt) {
@@ -1547,7 +1593,7 @@ __device__ __forceinline__ void rowcol_body(const gsbn_rowcol_args& a, const RcS
     for (int f = tid; f < nr * M; f += NT) {
       const int rr = f / M;
       const int j = f - rr * M;
-      const int aa = part + P * (base + rr);
+      const int aa = row_first + part + P * (base + rr);
       const int k = FUSED ? fl->rows[aa] : (base == 0 && f == tid) ? k_pre
                                                                    : a.rows_list[(int64_t)h * C + aa];
       const int64_t r = (int64_t)h * C + k;
@@ -1799,12 +1845,35 @@ template <int PREC>
 __global__ void __launch_bounds__(64, RC_WAVES(PREC))
     rowcol_kernel_batch_wave(const gsbn_rowcol_args* __restrict__ arr,
                              const int32_t* __restrict__ wg_proj,
-                             const int32_t* __restrict__ wg_hcu, int nhcu, gsbn_step_args st) {
-  for (int w = blockIdx.x; w < nhcu; w += gridDim.x) {
-    const gsbn_rowcol_args& a = arr[wg_proj[w]];
+                             const int32_t* __restrict__ wg_hcu, int nhcu, int nhelp,
+                             gsbn_step_args st) {
+  /* workgroups below nhelp (a multiple of the sub-queue count) run the
+   * queued rows of this step, helper i those of sub-queue i % 64; the others
+   * one hypercolumn each, keeping ovf_keep rows unless their sub-queue
+   * overflowed, in which case they run every row themselves */
+  const int capq = st.ovf_cap / GSBN_OVF_QUEUES;
+  if ((int)blockIdx.x < nhelp) {
+    const int qq = (int)blockIdx.x & (GSBN_OVF_QUEUES - 1);
+    const int n = st.ovf_cnt[ovf_counter(st.exec, qq)];
+    if (n > capq) return;
+    const int stride = nhelp / GSBN_OVF_QUEUES;
+    for (int i = (int)blockIdx.x / GSBN_OVF_QUEUES; i < n; i += stride) {
+      const int32_t* tk = st.ovf_task + 3 * ((int64_t)qq * capq + i);
+      const gsbn_rowcol_args& a = arr[tk[0]];
+      const RcStep q = {st.simstep, st.exec, 1, a.table_lens[st.kp_slot], a.taupdt * st.prn,
+                        a.tables[st.kp_slot]};
+      rowcol_body<PREC, false, true, 64>(a, q, tk[1], 0, nullptr, INT_MAX, tk[2], true);
+    }
+    return;
+  }
+  for (int w = (int)blockIdx.x - nhelp; w < nhcu; w += (int)gridDim.x - nhelp) {
+    const int pj = wg_proj[w], h = wg_hcu[w];
+    const gsbn_rowcol_args& a = arr[pj];
     const RcStep q = {st.simstep, st.exec, 1, a.table_lens[st.kp_slot], a.taupdt * st.prn,
                       a.tables[st.kp_slot]};
-    rowcol_body<PREC, false, true, 64>(a, q, wg_hcu[w], 0);
+    int keep = INT_MAX;
+    if (nhelp > 0 && st.ovf_cnt[ovf_counter(st.exec, ovf_queue(pj, h))] <= capq) keep = st.ovf_keep;
+    rowcol_body<PREC, false, true, 64>(a, q, h, 0, nullptr, keep);
   }
 }
 
@@ -2493,25 +2562,27 @@ int gsbn_proj_rowcol_batch_wave(const gsbn_rowcol_args* projs_dev, const int32_t
                                 const int32_t* wg_hcu, int32_t nhcu, const gsbn_step_args* step,
                                 void* stream) {
   if (!projs_dev || !wg_proj || !wg_hcu || !step || nhcu <= 0 || step->kp_slot < 0 ||
-      step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 || step->prec > GSBN_PREC_F16)
+      step->kp_slot >= GSBN_KP_SLOTS || step->prec < GSBN_PREC_F32 || step->prec > GSBN_PREC_F16 ||
+      (step->ovf_keep > 0 && (!step->ovf_cnt || !step->ovf_task || step->ovf_cap < 0)))
     return GSBN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   const dim3 b(64);
+  /* helper waves for the queued rows: enough to run a busy step's queue in
+   * about one pass of the device */
+  const int nhelp = step->ovf_keep > 0 ? GSBN_OVF_HELPERS : 0;
+  const unsigned g = (unsigned)(nhcu + nhelp);
   switch (step->prec) {
     case GSBN_PREC_F32:
-      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_F32>,
-                   dim3(loop_grid(rowcol_kernel_batch_wave<GSBN_PREC_F32>, 64, 0, nhcu)), b, 0, s,
-                   projs_dev, wg_proj, wg_hcu, nhcu, *step);
+      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_F32>, dim3(g), b, 0, s, projs_dev, wg_proj,
+                   wg_hcu, nhcu, nhelp, *step);
       break;
     case GSBN_PREC_BF16:
-      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_BF16>,
-                   dim3(loop_grid(rowcol_kernel_batch_wave<GSBN_PREC_BF16>, 64, 0, nhcu)), b, 0,
-                   s, projs_dev, wg_proj, wg_hcu, nhcu, *step);
+      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_BF16>, dim3(g), b, 0, s, projs_dev, wg_proj,
+                   wg_hcu, nhcu, nhelp, *step);
       break;
     default:
-      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_F16>,
-                   dim3(loop_grid(rowcol_kernel_batch_wave<GSBN_PREC_F16>, 64, 0, nhcu)), b, 0,
-                   s, projs_dev, wg_proj, wg_hcu, nhcu, *step);
+      launch_timed(rowcol_kernel_batch_wave<GSBN_PREC_F16>, dim3(g), b, 0, s, projs_dev, wg_proj,
+                   wg_hcu, nhcu, nhelp, *step);
       break;
   }
   return launch_status();

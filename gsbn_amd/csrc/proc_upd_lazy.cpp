@@ -1239,6 +1239,16 @@ class ProcUpdLazy : public ProcedureBase {
    * hypercolumn (gsbn_pop_update_batch_wave, _pop_upl units per lane) when
    * every population qualifies; 0: the block kernel */
   int _pop_wave = 1, _pop_upl = 0;
+  /* row overflow queue of the one-wave row/column launch (gsbn_step_args
+   * ovf_*): ovf-keep rows per hypercolumn in its own wave (0: no queue),
+   * ovf-cap queued rows at most per step (a busier step runs every row in
+   * the hypercolumns' own waves); _ovf_exec: the last step whose population
+   * launch cleared the next step's counter */
+  int _ovf_keep = 2, _ovf_cap = 1 << 18;
+  static constexpr int kOvfCounters = 2 * 64 * 16; /* 2 parities x 64 sub-queues, 64-byte stride */
+  int64_t _ovf_exec = -2;
+  DevBuf<int32_t> _ovf_cnt, _ovf_task;
+  vector<int32_t> _in_proj_host;
 
   int _max_m = 0, _parts0 = 1;
   /* Chained step (plan -2): one launch per step runs the previous step's
@@ -1435,6 +1445,10 @@ void ProcUpdLazy::init(SolverParam solver_param, Database& db, bool copy) {
   par.argi("rowcol-graded", _graded_fixed);
   par.argi("rowcol-wave", _wave_fixed);
   par.argi("pop-wave", _pop_wave);
+  par.argi("ovf-keep", _ovf_keep);
+  par.argi("ovf-cap", _ovf_cap);
+  if (_ovf_keep < 0 || _ovf_cap < 0)
+    throw Error(-2, "ProcUpdLazy: ovf-keep and ovf-cap must be >= 0");
   if (_wave_fixed < 0 || _wave_fixed > 4)
     throw Error(-2, "ProcUpdLazy: rowcol-wave must be in [0, 4]");
   par.argi("deferred-columns", _defer_arg);
@@ -1559,6 +1573,12 @@ void ProcUpdLazy::build_batch_maps() {
                                 : (int32_t)(std::find(_list_proj.begin(), _list_proj.end(),
                                                       p->_in[0]) - _list_proj.begin()));
   _in_proj.upload(in.data(), in.size());
+  _in_proj_host = in;
+  if (_pop_upl && _ovf_keep > 0) {
+    _ovf_cnt.alloc(kOvfCounters);
+    _ovf_task.alloc((size_t)3 * std::max(_ovf_cap, 1));
+    op_check(gsbn_i32_fill(_ovf_cnt.p, kOvfCounters, 0, active_stream()), "fill");
+  }
   _wg_pop.upload(wp.data(), wp.size());
   _wg_pop_hcu.upload(wh.data(), wh.size());
   if (!qp.empty()) {
@@ -1588,6 +1608,11 @@ void ProcUpdLazy::build_graded_map(int g) {
 void ProcUpdLazy::refresh_args(int simstep) {
   _pop_args.clear();
   for (auto* p : _list_pop) _pop_args.push_back(p->args(0, 0, 0.0f, 0, 0, true, _seed, _wta_pop_max));
+  /* the incoming projection's index in the batch's row/column argument
+   * array, which the overflow queue's tasks name */
+  if (_batch)
+    for (size_t i = 0; i < _pop_args.size() && i < _in_proj_host.size(); ++i)
+      _pop_args[i].proj0.bidx = _in_proj_host[i];
   _proj_args.clear();
   for (auto* q : _list_proj) _proj_args.push_back(q->static_args(_slot_prn, _nslots, (int64_t)simstep + 2));
   /* a host write (a new stimulus row, say) leaves these unchanged: the
@@ -1777,6 +1802,17 @@ void ProcUpdLazy::update_gpu() {
     if (plan == -1) plan = _parts0;
   }
   if (plan != -2 && _pending) flush_deferred();
+  /* the row overflow queue: passed to every one-wave population launch (it
+   * clears the next step's counter), used by the one-wave row/column plan */
+  const bool ovf = _batch && _pop_upl && _ovf_cnt.p;
+  if (ovf) {
+    st.ovf_cnt = _ovf_cnt.p;
+    st.ovf_task = _ovf_task.p;
+    st.ovf_cap = _ovf_cap;
+    st.ovf_keep = plan == -20 ? _ovf_keep : 0;
+    if (_ovf_exec != (int64_t)_exec - 1)
+      op_check(gsbn_i32_fill(_ovf_cnt.p, kOvfCounters, 0, stream), "fill"); /* not cleared by the last step */
+  }
   const size_t ev = 2 * (size_t)_tune_pos;
   if (timed && _run_len == 0) hip_check(hipEventRecord(_tev[ev], stream), "hipEventRecord");
 
@@ -1859,10 +1895,12 @@ void ProcUpdLazy::update_gpu() {
   } else {
     for (auto* p : _list_pop) p->spk_side = 0;
     Timing::get().begin(0, stream);
-    if (_batch && _pop_upl)
+    if (_batch && _pop_upl) {
       op_check(gsbn_pop_update_batch_wave(_pop_dev.p, _wg_pop.p, _wg_pop_hcu.p, _nwg_pop, _pop_upl,
                                           &st, stream),
                "gsbn_pop_update_batch_wave");
+      if (ovf) _ovf_exec = _exec;
+    }
     else if (_batch)
       op_check(gsbn_pop_update_batch(_pop_dev.p, (int32_t)_pop_args.size(), _wg_pop.p,
                                      _wg_pop_hcu.p, _nwg_pop, _pop_threads, _max_slots, _any_quirk,

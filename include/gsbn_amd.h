@@ -77,6 +77,20 @@ typedef struct {
   int32_t prec; /* trace form (GSBN_PREC_*) of every projection of a batched row/column launch */
   int64_t lginp_offset; /* added to every population's lginp pointer */
   int64_t wmask_offset; /* added to every population's wmask pointer */
+  /* Row overflow queue (ovf_keep > 0; gsbn_pop_update_batch_wave with
+   * gsbn_proj_rowcol_batch_wave): a hypercolumn whose active rows this step
+   * number more than ovf_keep keeps the first ovf_keep for its own row/column
+   * wave and queues the rest, one task per row, {proj0.bidx, hcu, list
+   * position}, in one of 64 sub-queues chosen by hashing (bidx, hcu): task i
+   * of sub-queue q at ovf_task[3 * (q * (ovf_cap / 64) + i)], i counted by
+   * ovf_cnt[((exec % 2) * 64 + q) * 16] (2 * 64 * 16 words; the population
+   * launch of step x clears the counters of parity x + 1; all start at 0).
+   * The row/column launch's helper waves run the tasks; when a sub-queue
+   * holds more than ovf_cap / 64 rows, its hypercolumns run all of their
+   * rows themselves. */
+  int32_t ovf_keep, ovf_cap;
+  int32_t* ovf_cnt;
+  int32_t* ovf_task;
 } gsbn_step_args;
 
 /* one projection entering a population, as seen by gsbn_pop_update */
@@ -114,7 +128,9 @@ typedef struct {
   int32_t* walk_list;
   int32_t* walk_cnt;
   int32_t* walk_T;
-  int32_t dK, dpad;
+  int32_t dK;
+  int32_t bidx; /* this projection's index in a batch's row/column argument array
+                   (the row overflow queue's tasks name it) */
 } gsbn_pop_proj;
 
 /* Delay queue in push form (Proj.cpp:645-664 with ProcExchangeSpike's

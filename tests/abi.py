@@ -14,7 +14,7 @@ class PopProj(ctypes.Structure):
                 ("kftj", f32), ("bgain", f32), ("eps", f32), ("fut", vp), ("W", i32),
                 ("rows_list", vp), ("rows_cnt", vp), ("act_stamp", vp), ("dhist", vp),
                 ("dcnt", vp), ("dfl", vp), ("walk_list", vp), ("walk_cnt", vp), ("walk_T", vp),
-                ("dK", i32), ("dpad", i32)]
+                ("dK", i32), ("bidx", i32)]
 
 
 class Fanout(ctypes.Structure):

@@ -267,14 +267,22 @@ def test_batch512_sampled_networks_bitwise(tmp_path):
     ora.close()
 
 
-@pytest.mark.parametrize("wave", [1, 2, 4])
-def test_batched_networks_wave_plan(tmp_path, wave):
-    # the batch's row/column update with one wave per hypercolumn (plan -20),
-    # or one wave per part of the graded splits g = 1 and 3 (plans -21, -23:
-    # the largest quarter of the hypercolumns in 2 resp. 8 waves)
+@pytest.mark.parametrize("wave,keep,cap", [(1, None, None), (1, 0, None), (1, 1, None),
+                                           (1, 1, 3), (1, 1, 128), (2, None, None),
+                                           (4, None, None)])
+def test_batched_networks_wave_plan(tmp_path, wave, keep, cap):
+    # the batch's row/column update with one wave per hypercolumn (plan -20)
+    # and its row overflow queue (rows past the first `keep` of a
+    # hypercolumn run in helper waves; cap 3: no room in any of the 64
+    # sub-queues, every row runs in its hypercolumn's own wave; cap 128: two
+    # rows per sub-queue, some overflow and some not; keep 0: no queue), or
+    # one wave per part of the graded splits g = 1 and 3 (plans -21, -23: the
+    # largest quarter of the hypercolumns in 2 resp. 8 waves)
     spec = configs.batched_nets(7, hcu_range=(3, 9), mcu_range=(3, 140), n_pat=4, train=40,
                                 quiet=5, recall=30, reset=20)
     spec.rowcol_wave = wave
+    spec.ovf_keep = keep
+    spec.ovf_cap = cap
     gpu, ora = _lockstep_and_compare(spec, tmp_path)
     c, t, flags = gpu.check()
     assert (c, t) == (ora.check.correct, ora.check.total)
