@@ -333,7 +333,10 @@ int gsbn_pop_update_batch_wave(const gsbn_pop_args* pops_dev, const int32_t* wg_
 
 /* gsbn_proj_rowcol_batch with one 64-thread workgroup (one wave) per
  * hypercolumn of the map: same results, more hypercolumns in flight when
- * each has little work. */
+ * each has little work.  With step->ovf_keep > 0 (after a
+ * gsbn_pop_update_batch_wave of the same step with the same queue), 4096
+ * helper waves ahead of them run the rows of the row overflow queue and each
+ * hypercolumn keeps ovf_keep rows (see gsbn_step_args). */
 int gsbn_proj_rowcol_batch_wave(const gsbn_rowcol_args* projs_dev, const int32_t* wg_proj,
                                 const int32_t* wg_hcu, int32_t nhcu, const gsbn_step_args* step,
                                 void* stream);
