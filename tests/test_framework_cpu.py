@@ -117,6 +117,8 @@ def test_solver_without_device_fails_loudly(tmp_path):
                                              ("rowcol_graded", "rowcol-graded", 2),
                                              ("rowcol_wave", "rowcol-wave", 3),
                                              ("pop_wave", "pop-wave", 0),
+                                             ("ovf_keep", "ovf-keep", 3),
+                                             ("ovf_cap", "ovf-cap", 4096),
                                              ("fused_step", "fused-step", 1),
                                              ("rowcol_parts", "rowcol-parts", 15),
                                              ("resident_stimulus", "resident-stimulus", 0)])
