@@ -267,10 +267,12 @@ def test_batch512_sampled_networks_bitwise(tmp_path):
     ora.close()
 
 
-@pytest.mark.parametrize("wave,keep,cap", [(1, None, None), (1, 0, None), (1, 1, None),
-                                           (1, 1, 3), (1, 1, 128), (2, None, None),
-                                           (4, None, None)])
-def test_batched_networks_wave_plan(tmp_path, wave, keep, cap):
+@pytest.mark.parametrize("wave,keep,cap,form", [(1, None, None, "fp32"), (1, 0, None, "fp32"),
+                                                (1, 1, None, "fp32"), (1, 1, 3, "fp32"),
+                                                (1, 1, 128, "fp32"), (1, 1, None, "bf16"),
+                                                (1, 1, None, "fp16"), (2, None, None, "fp32"),
+                                                (4, None, None, "fp32")])
+def test_batched_networks_wave_plan(tmp_path, wave, keep, cap, form):
     # the batch's row/column update with one wave per hypercolumn (plan -20)
     # and its row overflow queue (rows past the first `keep` of a
     # hypercolumn run in helper waves; cap 3: no room in any of the 64
@@ -283,6 +285,7 @@ def test_batched_networks_wave_plan(tmp_path, wave, keep, cap):
     spec.rowcol_wave = wave
     spec.ovf_keep = keep
     spec.ovf_cap = cap
+    spec.trace_precision = form
     gpu, ora = _lockstep_and_compare(spec, tmp_path)
     c, t, flags = gpu.check()
     assert (c, t) == (ora.check.correct, ora.check.total)
